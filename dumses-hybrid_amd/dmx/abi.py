@@ -1,0 +1,169 @@
+"""ctypes binding of libdmx.so (the C ABI declared in include/dmx.h).
+
+This is the Python host side of the drop-in boundary: the same entry points a
+Fortran iso_c_binding shim binds (see fortran/dmx_shim.f90), with the
+reference's argument meaning.  Loading fails loudly if the HIP library is not
+built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .problems import Params
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "build", "libdmx.so")
+
+
+class DmxParams(ctypes.Structure):
+    """Mirror of ``struct dmx_params`` (include/dmx.h)."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("iso", ctypes.c_int32), ("slope_type", ctypes.c_int32),
+        ("riemann", ctypes.c_int32), ("riemann2d", ctypes.c_int32),
+        ("bdtype", ctypes.c_int32 * 3), ("fargo", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+        ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+        ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("ciso", ctypes.c_double),
+        ("omega0", ctypes.c_double), ("courant", ctypes.c_double),
+    ]
+
+
+def params_to_c(p: Params) -> DmxParams:
+    c = DmxParams()
+    for k in ("nx", "ny", "nz", "iso", "slope_type", "riemann", "riemann2d", "fargo",
+              "precision"):
+        setattr(c, k, int(getattr(p, k)))
+    for i in range(3):
+        c.bdtype[i] = int(p.bdtype[i])
+    for k in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "gamma", "ciso", "omega0",
+              "courant"):
+        setattr(c, k, float(getattr(p, k)))
+    return c
+
+
+# every entry point include/dmx.h declares
+EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compute_dt",
+           "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_advance", "dmx_history",
+           "dmx_last_advance_ms", "dmx_last_error", "dmx_version")
+
+_lib = None
+
+
+class DmxError(RuntimeError):
+    pass
+
+
+def lib(path: str | None = None):
+    """Loads libdmx.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or os.environ.get("DMX_LIB", LIB_PATH)
+    if not os.path.exists(path):
+        raise DmxError(f"libdmx.so not built at {path}: run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    vp = ctypes.c_void_p
+    L.dmx_create.argtypes = [ctypes.POINTER(DmxParams), ctypes.POINTER(vp)]
+    L.dmx_destroy.argtypes = [vp]
+    L.dmx_destroy.restype = None
+    L.dmx_upload.argtypes = [vp, dp]
+    L.dmx_download.argtypes = [vp, dp]
+    L.dmx_compute_dt.argtypes = [vp, dp]
+    L.dmx_godunov.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    L.dmx_fargo.argtypes = [vp, ctypes.c_double]
+    L.dmx_boundary.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    L.dmx_advance.argtypes = [vp, ctypes.c_int32, dp, dp]
+    L.dmx_history.argtypes = [vp, dp]
+    L.dmx_last_advance_ms.argtypes = [vp, dp]
+    L.dmx_last_error.restype = ctypes.c_char_p
+    L.dmx_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().dmx_last_error()
+        raise DmxError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+class Domain:
+    """One device-resident ghosted domain (a dmx_ctx)."""
+
+    def __init__(self, p: Params, uin: np.ndarray | None = None):
+        self.p = p
+        self._cp = params_to_c(p)
+        h = ctypes.c_void_p()
+        _check(lib().dmx_create(ctypes.byref(self._cp), ctypes.byref(h)), "dmx_create")
+        self.h = h
+        self.time = 0.0
+        if uin is not None:
+            self.upload(uin)
+
+    def upload(self, uin: np.ndarray):
+        u = np.ascontiguousarray(uin, dtype=np.float64)
+        p = self.p
+        if u.shape != (11, p.nz + 6, p.ny + 6, p.nx + 6):
+            raise DmxError(f"uin shape {u.shape} does not match the domain")
+        _check(lib().dmx_upload(self.h, _ptr(u)), "dmx_upload")
+
+    def download(self) -> np.ndarray:
+        p = self.p
+        u = np.zeros((11, p.nz + 6, p.ny + 6, p.nx + 6))
+        _check(lib().dmx_download(self.h, _ptr(u)), "dmx_download")
+        return u
+
+    def compute_dt(self) -> float:
+        dt = ctypes.c_double()
+        _check(lib().dmx_compute_dt(self.h, ctypes.byref(dt)), "dmx_compute_dt")
+        return dt.value
+
+    def godunov(self, time: float, dt: float):
+        _check(lib().dmx_godunov(self.h, time, dt), "dmx_godunov")
+
+    def fargo(self, dt: float):
+        _check(lib().dmx_fargo(self.h, dt), "dmx_fargo")
+
+    def boundary(self, time: float, dt: float):
+        _check(lib().dmx_boundary(self.h, time, dt), "dmx_boundary")
+
+    def advance(self, nsteps: int, time: float | None = None, want_dts: bool = True):
+        t = ctypes.c_double(self.time if time is None else time)
+        dts = np.zeros(max(nsteps, 1))
+        _check(lib().dmx_advance(self.h, nsteps, ctypes.byref(t),
+                                 _ptr(dts) if want_dts else None), "dmx_advance")
+        self.time = t.value
+        return t.value, dts[:nsteps]
+
+    def history(self) -> np.ndarray:
+        out = np.zeros(9)
+        _check(lib().dmx_history(self.h, _ptr(out)), "dmx_history")
+        return out
+
+    def last_advance_ms(self) -> float:
+        ms = ctypes.c_double()
+        _check(lib().dmx_last_advance_ms(self.h, ctypes.byref(ms)), "dmx_last_advance_ms")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dmx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

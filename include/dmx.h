@@ -1,0 +1,133 @@
+/*
+ * dmx.h -- C ABI of the MI355X-native DUMSES-Hybrid MHD step (libdmx.so).
+ *
+ * One context owns one device-resident ghosted domain (nghost = 3) and runs the
+ * reference's second-order Godunov MHD step on it: MUSCL-Hancock slopes and
+ * trace, 1D (HLLD/HLL/LLF/upwind) face fluxes, 2D (HLLD/LLF/upwind/HLLA) edge
+ * EMFs with constrained transport, CFL timestep, ghost fills (periodic,
+ * zero-gradient, shearing-periodic x) and FARGO orbital advection.
+ *
+ * The entry points replace the reference's external Fortran subroutines
+ * (no explicit interface, all state in module globals), which a Fortran
+ * iso_c_binding shim forwards here:
+ *
+ *   dmx_compute_dt  <- compute_dt(dt)           src/subroutines/physics.f90:18
+ *   dmx_godunov     <- godunov                  src/subroutines/scheme.f90:104
+ *                      (includes update + constrained_transport,
+ *                       src/subroutines/scheme.f90:1764 and :1940)
+ *   dmx_fargo       <- fargo_update             src/subroutines/fargo.f90:17
+ *   dmx_boundary    <- boundary                 src/subroutines/boundary.f90:18
+ *   dmx_upload      <- the host->device copy of uin at the data region
+ *                      (src/dumses.f90:59, after init / restart_run)
+ *   dmx_download    <- "!$acc update host(uin)" before output/history/special
+ *                      (src/dumses.f90:62,122,133,138,144)
+ *   dmx_history     <- the FP64 volume sums of history
+ *                      (src/subroutines/history.f90:62-118)
+ *
+ * Conventions (the reference has none: it stops or ignores errors):
+ *   - every function returns 0 on success and a negative DMX_E* code on error;
+ *   - dmx_last_error() returns a message for the last error of the thread;
+ *   - calls are synchronous at the ABI (internally asynchronous on one HIP
+ *     stream); a context is not thread-safe, as the reference's module state.
+ *
+ * Arrays crossing the ABI use the reference layout of uin:
+ *   uin(iu1:iu2, ju1:ju2, ku1:ku2, 1:11), Fortran column-major (i fastest,
+ *   variable slowest), iu1 = -2, iu2 = nx+3 (src/subroutines/init.f90:34-42,
+ *   :357), variables rho, rho*u, rho*v, rho*w, E, Bx_L, By_L, Bz_L, Bx_R, By_R,
+ *   Bz_R (src/modules/commons.f90:72-80).
+ */
+#ifndef DMX_H
+#define DMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Riemann solver ids (src/modules/commons.f90:104-111) */
+#define DMX_LLF     2
+#define DMX_HLL     3
+#define DMX_HLLD    4
+#define DMX_UPWIND  5
+#define DMX_HLLA    8
+
+/* boundary types (the bdtype{x,y,z} strings, src/modules/commons.f90:128-130) */
+#define DMX_BC_PERIODIC       0
+#define DMX_BC_ZERO_GRADIENT  1
+#define DMX_BC_SHEARINGBOX    2   /* x only */
+
+/* arithmetic of the step */
+#define DMX_FP64  0   /* everything in float64 (the reference's arithmetic) */
+#define DMX_FP32  1   /* bulk float32; float64 face-B/CT, pressure recovery, sums */
+
+/* error codes */
+#define DMX_OK          0
+#define DMX_E_PARAM    -1   /* invalid parameter (unknown solver, size, bc) */
+#define DMX_E_NOMEM    -2   /* device allocation failed */
+#define DMX_E_HIP      -3   /* HIP runtime error */
+#define DMX_E_NONFINITE -4  /* dt not finite or not positive */
+#define DMX_E_STATE    -5   /* call out of order (e.g. step before upload) */
+
+/* Parameters of the reference namelists scheme_params/model_params
+ * (src/subroutines/init.f90:153-173, defaults :264-299). */
+typedef struct dmx_params {
+    int32_t nx, ny, nz;          /* interior cells, each >= 3                 */
+    int32_t iso;                 /* 1: isothermal (reference ISO=1 build)     */
+    int32_t slope_type;          /* 0 none, 1 minmod, 2 MC, 3 multidim        */
+    int32_t riemann;             /* DMX_LLF | DMX_HLL | DMX_HLLD | DMX_UPWIND  */
+    int32_t riemann2d;           /* DMX_LLF | DMX_HLLD | DMX_UPWIND | DMX_HLLA */
+    int32_t bdtype[3];           /* DMX_BC_* per direction                    */
+    int32_t fargo;               /* 1: FARGO orbital advection (needs omega0) */
+    int32_t precision;           /* DMX_FP64 | DMX_FP32                       */
+    double  xmin, xmax, ymin, ymax, zmin, zmax;
+    double  gamma, ciso, omega0, courant;
+} dmx_params;
+
+typedef struct dmx_ctx dmx_ctx;
+
+/* Creates a context on the current HIP device and allocates the domain.
+ * Replaces allocate_workspace (src/subroutines/init.f90:348) for the step. */
+int  dmx_create(const dmx_params* p, dmx_ctx** ctx);
+void dmx_destroy(dmx_ctx* ctx);
+
+/* Host <-> device copies of the full ghosted uin (reference layout above). */
+int  dmx_upload(dmx_ctx* ctx, const double* uin);
+int  dmx_download(dmx_ctx* ctx, double* uin);
+
+/* CFL timestep: dt = courant * min 1/(cfx/dx + cfy/dy + cfz/dz) over the
+ * interior (src/subroutines/physics.f90:18-132). */
+int  dmx_compute_dt(dmx_ctx* ctx, double* dt);
+
+/* One Godunov step of length dt at time `time` (time enters the shearing-box
+ * flux/EMF remap at time + dt/2, src/subroutines/scheme.f90:176-182). */
+int  dmx_godunov(dmx_ctx* ctx, double time, double dt);
+
+/* FARGO orbital advection over dt (src/subroutines/fargo.f90:17-124). */
+int  dmx_fargo(dmx_ctx* ctx, double dt);
+
+/* Ghost fill at the end of a step (shearing remap at time + dt,
+ * src/subroutines/boundary.f90:18-92). */
+int  dmx_boundary(dmx_ctx* ctx, double time, double dt);
+
+/* Runs nsteps of the reference time loop (src/dumses.f90:75-120) on the
+ * device without host synchronisation per step: compute_dt, godunov,
+ * [fargo], boundary, time += dt.  *time is read and updated; if dts is not
+ * NULL it receives the nsteps timesteps. */
+int  dmx_advance(dmx_ctx* ctx, int32_t nsteps, double* time, double* dts);
+
+/* FP64 volume sums of src/subroutines/history.f90:62-118, in the column order
+ * of history.txt after time and dt: mass, maxwell, reynolds, maxwell+reynolds,
+ * magp, meanBx, meanBy, meanBz, divB. */
+int  dmx_history(dmx_ctx* ctx, double out[9]);
+
+/* Synchronise and time: milliseconds of the last dmx_advance on the device. */
+int  dmx_last_advance_ms(dmx_ctx* ctx, double* ms);
+
+const char* dmx_last_error(void);
+const char* dmx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMX_H */

@@ -1,0 +1,230 @@
+"""Pins the oracle (the C restatement) before it is trusted as the checker.
+
+1. Against the reference's own outputs: the golden fixtures written from the
+   compiled reference (tests/golden/make_golden.py) must be reproduced bit for
+   bit, function by function (Riemann solvers) and step by step.
+2. Against what the mathematics fixes, independently of the reference:
+   consistency of the Riemann solvers, uniform-state invariance, discrete
+   div B conservation by constrained transport, periodic conservation of mass
+   and energy, the analytic CFL step, exact integer-cell FARGO shifts and the
+   shearing boundary at zero shift.
+3. Live, where the reference was built in this container: fresh random cases.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dmx import problems as P
+from goldens import GOLDEN, STEP_FIXTURES, load_step, params_from_json
+
+pytestmark = pytest.mark.skipif(not os.path.exists(O.LIB_PATH),
+                                reason="oracle library not built (make -C oracle)")
+
+
+# --------------------------------------------------------------------------
+# 1. reference-held vectors
+# --------------------------------------------------------------------------
+def test_riemann_solvers_match_reference_bitwise():
+    z = np.load(os.path.join(GOLDEN, "riemann_kernels.npz"))
+    s1, s2 = z["s1"], z["s2"]
+    g, c, dt = float(z["gamma"]), float(z["ciso"]), float(z["dt"])
+    for iso in (0, 1):
+        for name, solver in (("hlld", P.HLLD), ("hll", P.HLL), ("llf", P.LLF),
+                             ("upwind", P.UPWIND)):
+            want = z[f"iso{iso}_{name}"]
+            for n in range(s1.shape[0]):
+                f, st = O.riemann1d(solver, s1[n, :8], s1[n, 8:], g, c, iso)
+                got = np.concatenate([f, st]) if name == "hlld" else f
+                assert np.array_equal(got, want[n], equal_nan=True), (iso, name, n, got, want[n])
+        emf = z[f"iso{iso}_emf"]
+        for col, solver in enumerate((P.HLLD, P.LLF, P.UPWIND, P.HLLA)):
+            for n in range(s2.shape[0]):
+                e = O.riemann2d(solver, s2[n].reshape(4, 8), dt, g, c, iso)
+                assert e == emf[n, col] or (np.isnan(e) and np.isnan(emf[n, col])), \
+                    (iso, solver, n, e, emf[n, col])
+
+
+@pytest.mark.parametrize("name", STEP_FIXTURES)
+def test_step_matches_reference_bitwise(name):
+    p, u, dts, outs, full = load_step(name)
+    o = O.Oracle(p, u)
+    t = 0.0
+    for s in range(1, int(max(outs)) + 1):
+        t, d = o.advance(1, t)
+        assert d[0] == dts[s - 1], (name, s, d[0], dts[s - 1])
+        if s in outs:
+            got = P.interior(o.store())
+            assert np.array_equal(got, outs[s]), (name, s, np.abs(got - outs[s]).max())
+    if full is not None:
+        assert np.array_equal(o.store(), full)
+
+
+def test_magnetic_loop_200_steps_matches_reference():
+    z = np.load(os.path.join(GOLDEN, "magnetic_loop_200.npz"))
+    p = params_from_json(z["params"])
+    pb = P.magnetic_loop()
+    assert pb.params.nx == p.nx
+    o = O.Oracle(p, pb.uin)
+    _, dts = o.advance(200)
+    assert np.array_equal(dts, z["dts"])
+    inner = np.ascontiguousarray(P.interior(o.store()))
+    assert np.array_equal(inner[:, :, ::4, ::4], z["sub"])
+    assert hashlib.sha256(inner.tobytes()).hexdigest() == str(z["sha256"])
+    assert np.array_equal(o.history(), z["history"])
+
+
+# --------------------------------------------------------------------------
+# 2. closed forms and invariants
+# --------------------------------------------------------------------------
+def _phys_flux(q, gamma):
+    """Ideal-MHD flux of one state [rho, p, u, a, v, b, w, c] (normal x)."""
+    r, pr, u, a, v, b, w, c = q
+    em = 0.5 * (a * a + b * b + c * c)
+    et = pr / (gamma - 1) + 0.5 * r * (u * u + v * v + w * w) + em
+    pt = pr + em
+    return np.array([r * u, (et + pt) * u - a * (u * a + v * b + w * c),
+                     r * u * u + pt - a * a, 0.0, r * u * v - a * b, b * u - a * v,
+                     r * u * w - a * c, c * u - a * w]), pt
+
+
+@pytest.mark.parametrize("solver", [P.HLLD, P.HLL, P.LLF, P.UPWIND])
+def test_riemann1d_consistency(solver):
+    """F(U, U) equals the physical flux (HLLD returns p_T separately)."""
+    g = np.random.default_rng(4)
+    for _ in range(200):
+        q = np.array([np.exp(g.uniform(-1, 1)), np.exp(g.uniform(-1, 1)), *g.normal(0, 1, 6)])
+        f, st = O.riemann1d(solver, q, q, 5 / 3)
+        want, pt = _phys_flux(q, 5 / 3)
+        if solver == P.HLLD:
+            want[2] -= pt
+            assert st[6] == pytest.approx(pt, rel=1e-13)
+        np.testing.assert_allclose(f, want, rtol=1e-12, atol=1e-12)
+
+
+def test_hlld_supersonic_upwinding():
+    """With both fast waves moving right the HLLD flux is the left flux."""
+    ql = np.array([1.0, 0.5, 10.0, 0.3, 0.2, 0.4, -0.1, 0.2])
+    qr = np.array([0.5, 0.3, 10.5, 0.3, -0.2, 0.1, 0.3, -0.5])
+    f, st = O.riemann1d(P.HLLD, ql, qr, 5 / 3)
+    want, pt = _phys_flux(ql, 5 / 3)
+    want[2] -= pt
+    np.testing.assert_allclose(f, want, rtol=1e-14)
+
+
+@pytest.mark.parametrize("solver", [P.HLLD, P.LLF, P.UPWIND, P.HLLA])
+def test_riemann2d_uniform_state(solver):
+    """Four equal corner states give E = (u B - v A) dt."""
+    g = np.random.default_rng(5)
+    for _ in range(100):
+        r, pr = np.exp(g.uniform(-1, 1, 2))
+        u, v, A, B, C = g.normal(0, 1, 5)
+        q = np.array([r, pr, u, v, A, B, C, u * B - v * A])
+        e = O.riemann2d(solver, np.stack([q] * 4), 0.01, 5 / 3)
+        assert e == pytest.approx((u * B - v * A) * 0.01, rel=1e-12, abs=1e-15)
+
+
+def _divb(u, p):
+    bx, by, bz = u[5], u[6], u[7]
+    s = (slice(3, -3),) * 3
+    d = ((bx[3:-3, 3:-3, 4:-2] - bx[s]) / p.dx + (by[3:-3, 4:-2, 3:-3] - by[s]) / p.dy
+         + (bz[4:-2, 3:-3, 3:-3] - bz[s]) / p.dz)
+    return d
+
+
+@pytest.mark.parametrize("riemann2d", [P.HLLD, P.LLF, P.UPWIND, P.HLLA])
+def test_ct_preserves_divb_and_conserves(riemann2d):
+    pb = P.random_smooth(12, 10, 8, seed=9)
+    p = pb.params.replace(riemann2d=riemann2d)
+    o = O.Oracle(p, pb.uin)
+    u0 = o.store()
+    b0 = np.abs(_divb(u0, p)).max()
+    o.advance(6)
+    u1 = o.store()
+    bmax = np.abs(u1[5:8]).max()
+    assert np.abs(_divb(u1, p)).max() <= b0 + 1e-13 * bmax / min(p.dx, p.dy, p.dz)
+    for v in (0, 4, 1, 2, 3):  # mass, energy, momentum in a periodic box
+        s0 = P.interior(u0)[v].sum()
+        s1 = P.interior(u1)[v].sum()
+        assert s1 == pytest.approx(s0, rel=1e-12, abs=1e-12 * np.abs(P.interior(u0)[v]).sum())
+
+
+def test_uniform_state_is_invariant_and_dt_analytic():
+    p = P.Params(nx=8, ny=6, nz=5, xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
+    rho, pr, v, b = 1.3, 0.7, (0.4, -0.2, 0.1), (0.3, 0.5, -0.2)
+    sh = (p.nz, p.ny, p.nx)
+    u = P.assemble(p, np.full(sh, rho), np.full(sh, v[0]), np.full(sh, v[1]), np.full(sh, v[2]),
+                   np.full(sh, pr), np.full(sh, b[0]), np.full(sh, b[1]), np.full(sh, b[2]))
+    o = O.Oracle(p, u)
+    dt = o.compute_dt()
+    g = p.gamma
+    c2, b2 = g * pr / rho, sum(x * x for x in b)
+    d2 = 0.5 * (c2 + b2 / rho)
+    cf = [np.sqrt(d2 + np.sqrt(d2 * d2 - c2 * bn * bn / rho)) + abs(vn) for bn, vn in zip(b, v)]
+    want = p.courant / (cf[0] / p.dx + cf[1] / p.dy + cf[2] / p.dz)
+    assert dt == pytest.approx(want, rel=1e-14)
+    o.advance(3)
+    np.testing.assert_allclose(o.store(), u, rtol=1e-13, atol=1e-15)
+
+
+def test_fargo_integer_shift_is_exact_translation():
+    """With |vK dt| a whole number of cells the FARGO remap translates each
+    x-column exactly by that number of cells (the slope terms vanish at
+    eps = 0) and keeps the y-sums."""
+    pb = P.random_smooth(6, 12, 4, seed=2, xmin=-0.5, xmax=0.5, ymin=-2.0, ymax=2.0)
+    p = pb.params.replace(omega0=1.0, fargo=1, bdtype=(P.SHEARINGBOX, P.PERIODIC, P.PERIODIC))
+    o = O.Oracle(p, pb.uin)
+    dt = 2.0 * p.dy / (1.5 * abs(-0.5 + p.dx / 2))  # two cells at the first column
+    before = o.store()
+    o.fargo(dt)
+    after = o.store()
+    i = 3  # array index of x-cell 1
+    vk = -1.5 * 1.0 * (-0.5 + p.dx / 2)
+    nshift = int(round(abs(vk * dt) / p.dy))
+    assert nshift == 2
+    col0 = before[0:5, 3:-3, 3:-3, i]
+    col1 = after[0:5, 3:-3, 3:-3, i]
+    np.testing.assert_allclose(col1, np.roll(col0, nshift, axis=2), rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(P.interior(after)[0:5].sum(axis=2), P.interior(before)[0:5].sum(axis=2),
+                               rtol=1e-13)
+
+
+def test_shearing_boundary_at_zero_shift_is_periodic():
+    pb = P.random_smooth(6, 8, 5, seed=4, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0)
+    pper = pb.params
+    pshr = pper.replace(omega0=1.0, bdtype=(P.SHEARINGBOX, P.PERIODIC, P.PERIODIC))
+    a = O.Oracle(pper, pb.uin)
+    b = O.Oracle(pshr, pb.uin)
+    a.boundary(0.0, 0.0)
+    b.boundary(0.0, 0.0)
+    ua, ub = a.store(), b.store()
+    # the shearing fill leaves the first outer left face (cell nx+1) as it was
+    ub[5, :, :, pper.nx + 3] = ua[5, :, :, pper.nx + 3]
+    # inner-ghost By is rebuilt as By(j-1) + 1*(By(j) - By(j-1)) (the forward
+    # y-slope at eps = 1, shearingbox.f90:92-93): equal to By(j) up to rounding
+    by = (6, 9)
+    scale = np.abs(ua[by, :, :, :3]).max()
+    np.testing.assert_allclose(ua[by, :, :, :3], ub[by, :, :, :3], rtol=0, atol=2.3e-16 * scale)
+    ub[by, :, :, :3] = ua[by, :, :, :3]
+    np.testing.assert_array_equal(ua, ub)
+
+
+# --------------------------------------------------------------------------
+# 3. live comparisons with the compiled reference (build container only)
+# --------------------------------------------------------------------------
+@pytest.mark.skipif(not O.reference_available(0), reason="reference not built here")
+@pytest.mark.parametrize("seed", [21, 22])
+def test_live_random_cases_match_reference(seed):
+    g = np.random.default_rng(seed)
+    pb = P.random_smooth(int(g.integers(5, 10)), int(g.integers(5, 10)), int(g.integers(5, 9)),
+                         seed=seed)
+    p = pb.params.replace(riemann=int(g.choice([2, 3, 4, 5])),
+                          riemann2d=int(g.choice([2, 4, 5, 8])),
+                          slope_type=int(g.integers(0, 4)))
+    ref, dts, _ = O.run_reference(p, pb.uin, 3)
+    o = O.Oracle(p, pb.uin)
+    _, odts = o.advance(3)
+    assert np.array_equal(dts, odts)
+    assert np.array_equal(o.store(), ref)
