@@ -1,0 +1,170 @@
+// dmx_abi.cpp -- the extern "C" entry points of include/dmx.h.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "dmx_engine.h"
+
+namespace dmx {
+static thread_local char g_err[512] = "";
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+} // namespace dmx
+
+struct dmx_ctx {
+    dmx::Engine* e;
+};
+
+static int validate(const dmx_params* p)
+{
+    using dmx::set_error;
+    if (!p) { set_error("null params"); return DMX_E_PARAM; }
+    if (p->nx < 3 || p->ny < 3 || p->nz < 3) {
+        set_error("nx, ny, nz must be >= 3 (periodic ghost copies read cells n-2..n)");
+        return DMX_E_PARAM;
+    }
+    if (p->riemann != DMX_LLF && p->riemann != DMX_HLL && p->riemann != DMX_HLLD && p->riemann != DMX_UPWIND) {
+        set_error("unknown 1D Riemann solver id %d", p->riemann);
+        return DMX_E_PARAM;
+    }
+    if (p->riemann2d != DMX_LLF && p->riemann2d != DMX_HLLD && p->riemann2d != DMX_UPWIND && p->riemann2d != DMX_HLLA) {
+        set_error("unknown 2D Riemann solver id %d", p->riemann2d);
+        return DMX_E_PARAM;
+    }
+    if (p->slope_type < 0 || p->slope_type > 3) {
+        set_error("unknown slope type %d", p->slope_type);
+        return DMX_E_PARAM;
+    }
+    for (int d = 0; d < 3; d++) {
+        int b = p->bdtype[d];
+        if (b != DMX_BC_PERIODIC && b != DMX_BC_ZERO_GRADIENT && !(b == DMX_BC_SHEARINGBOX && d == 0)) {
+            set_error("unsupported boundary type %d in direction %d (analytic boundaries are host callbacks)", b, d);
+            return DMX_E_PARAM;
+        }
+    }
+    if (p->bdtype[0] == DMX_BC_SHEARINGBOX && p->bdtype[1] != DMX_BC_PERIODIC) {
+        set_error("a shearing-box x boundary needs a periodic y boundary");
+        return DMX_E_PARAM;
+    }
+    if (p->omega0 < 0.0) { set_error("omega0 must be >= 0"); return DMX_E_PARAM; }
+    if (p->fargo && !(p->omega0 > 0.0)) { set_error("fargo needs omega0 > 0"); return DMX_E_PARAM; }
+    if (!(p->gamma > 1.0)) { set_error("gamma must be > 1"); return DMX_E_PARAM; }
+    if (!(p->courant > 0.0)) { set_error("courant must be > 0"); return DMX_E_PARAM; }
+    if (!(p->xmax > p->xmin) || !(p->ymax > p->ymin) || !(p->zmax > p->zmin)) {
+        set_error("empty domain");
+        return DMX_E_PARAM;
+    }
+    if (p->precision != DMX_FP64 && p->precision != DMX_FP32) {
+        set_error("unknown precision %d", p->precision);
+        return DMX_E_PARAM;
+    }
+    return DMX_OK;
+}
+
+extern "C" {
+
+int dmx_create(const dmx_params* p, dmx_ctx** ctx)
+{
+    if (!ctx) { dmx::set_error("null ctx"); return DMX_E_PARAM; }
+    *ctx = nullptr;
+    int rc = validate(p);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        dmx::set_error("no HIP device");
+        return DMX_E_HIP;
+    }
+    dmx::Engine* e = p->precision == DMX_FP32 ? dmx::make_engine_f32(*p, &rc) : dmx::make_engine_f64(*p, &rc);
+    if (!e) return rc ? rc : DMX_E_HIP;
+    *ctx = new dmx_ctx{e};
+    return DMX_OK;
+}
+
+void dmx_destroy(dmx_ctx* ctx)
+{
+    if (!ctx) return;
+    delete ctx->e;
+    delete ctx;
+}
+
+#define CTX_CHECK(c)                                              \
+    do {                                                          \
+        if (!(c) || !(c)->e) {                                    \
+            dmx::set_error("null context");                       \
+            return DMX_E_STATE;                                   \
+        }                                                         \
+    } while (0)
+
+int dmx_upload(dmx_ctx* c, const double* uin)
+{
+    CTX_CHECK(c);
+    if (!uin) { dmx::set_error("null uin"); return DMX_E_PARAM; }
+    return c->e->upload(uin);
+}
+int dmx_download(dmx_ctx* c, double* uin)
+{
+    CTX_CHECK(c);
+    if (!uin) { dmx::set_error("null uin"); return DMX_E_PARAM; }
+    return c->e->download(uin);
+}
+int dmx_compute_dt(dmx_ctx* c, double* dt)
+{
+    CTX_CHECK(c);
+    if (!dt) { dmx::set_error("null dt"); return DMX_E_PARAM; }
+    return c->e->compute_dt(dt);
+}
+int dmx_godunov(dmx_ctx* c, double time, double dt)
+{
+    CTX_CHECK(c);
+    if (!(dt > 0.0)) { dmx::set_error("dt must be > 0"); return DMX_E_PARAM; }
+    return c->e->godunov(time, dt);
+}
+int dmx_fargo(dmx_ctx* c, double dt)
+{
+    CTX_CHECK(c);
+    return c->e->fargo(dt);
+}
+int dmx_boundary(dmx_ctx* c, double time, double dt)
+{
+    CTX_CHECK(c);
+    return c->e->boundary(time, dt);
+}
+int dmx_advance(dmx_ctx* c, int32_t nsteps, double* time, double* dts)
+{
+    CTX_CHECK(c);
+    if (!time) { dmx::set_error("null time"); return DMX_E_PARAM; }
+    return c->e->advance(nsteps, time, dts);
+}
+int dmx_history(dmx_ctx* c, double out[9])
+{
+    CTX_CHECK(c);
+    if (!out) { dmx::set_error("null out"); return DMX_E_PARAM; }
+    return c->e->history(out);
+}
+int dmx_last_advance_ms(dmx_ctx* c, double* ms)
+{
+    CTX_CHECK(c);
+    if (!ms) { dmx::set_error("null ms"); return DMX_E_PARAM; }
+    return c->e->last_advance_ms(ms);
+}
+int dmx_profile(dmx_ctx* c, int32_t nsteps, double* time, double* ms, int32_t nkernels)
+{
+    CTX_CHECK(c);
+    if (!time || !ms || nkernels < 0) { dmx::set_error("bad profile arguments"); return DMX_E_PARAM; }
+    return c->e->profile(nsteps, time, ms, nkernels);
+}
+const char* dmx_kernel_name(int32_t id)
+{
+    return (id >= 0 && id < dmx::KG_N) ? dmx::kKernelGroupNames[id] : "";
+}
+const char* dmx_last_error(void) { return dmx::g_err; }
+const char* dmx_version(void) { return "dmx 0.1 (gfx950)"; }
+
+} // extern "C"

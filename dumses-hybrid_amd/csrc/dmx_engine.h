@@ -1,0 +1,41 @@
+// dmx_engine.h -- host-side engine interface behind the C ABI.
+//
+// One Engine owns one device-resident domain and one HIP stream.  The float64
+// and float32 engines are built from the same kernel source in two
+// translation units (dmx_f64.hip, dmx_f32.hip); dmx_abi.cpp picks one per
+// context from dmx_params.precision.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/dmx.h"
+
+namespace dmx {
+
+class Engine {
+public:
+    virtual ~Engine() {}
+    virtual int upload(const double* uin) = 0;
+    virtual int download(double* uin) = 0;
+    virtual int compute_dt(double* dt) = 0;
+    virtual int godunov(double time, double dt) = 0;
+    virtual int fargo(double dt) = 0;
+    virtual int boundary(double time, double dt) = 0;
+    virtual int advance(int nsteps, double* time, double* dts) = 0;
+    virtual int history(double out[9]) = 0;
+    virtual int last_advance_ms(double* ms) = 0;
+    virtual int profile(int nsteps, double* time, double* ms, int nk) = 0;
+};
+
+// kernel groups timed by dmx_profile
+enum KernelGroup { KG_DT = 0, KG_TRACE, KG_FACES, KG_EDGES, KG_SHEARFLUX, KG_UPDATE, KG_FARGO, KG_BOUNDARY, KG_N };
+static const char* const kKernelGroupNames[KG_N] = {
+    "compute_dt", "trace", "faces", "edges", "shearing_flux", "update_ct", "fargo", "boundary"};
+
+// set by the engines, read by dmx_last_error()
+void set_error(const char* fmt, ...);
+
+Engine* make_engine_f64(const dmx_params& p, int* rc);
+Engine* make_engine_f32(const dmx_params& p, int* rc);
+
+} // namespace dmx

@@ -50,7 +50,8 @@ def params_to_c(p: Params) -> DmxParams:
 # every entry point include/dmx.h declares
 EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compute_dt",
            "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_advance", "dmx_history",
-           "dmx_last_advance_ms", "dmx_last_error", "dmx_version")
+           "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_last_error",
+           "dmx_version")
 
 _lib = None
 
@@ -82,6 +83,9 @@ def lib(path: str | None = None):
     L.dmx_advance.argtypes = [vp, ctypes.c_int32, dp, dp]
     L.dmx_history.argtypes = [vp, dp]
     L.dmx_last_advance_ms.argtypes = [vp, dp]
+    L.dmx_profile.argtypes = [vp, ctypes.c_int32, dp, dp, ctypes.c_int32]
+    L.dmx_kernel_name.argtypes = [ctypes.c_int32]
+    L.dmx_kernel_name.restype = ctypes.c_char_p
     L.dmx_last_error.restype = ctypes.c_char_p
     L.dmx_version.restype = ctypes.c_char_p
     _lib = L
@@ -156,6 +160,14 @@ class Domain:
         ms = ctypes.c_double()
         _check(lib().dmx_last_advance_ms(self.h, ctypes.byref(ms)), "dmx_last_advance_ms")
         return ms.value
+
+    def profile(self, nsteps: int, time: float | None = None, ngroups: int = 8) -> dict:
+        """Mean device ms per step of each kernel group over nsteps."""
+        t = ctypes.c_double(self.time if time is None else time)
+        ms = np.zeros(ngroups)
+        _check(lib().dmx_profile(self.h, nsteps, ctypes.byref(t), _ptr(ms), ngroups), "dmx_profile")
+        self.time = t.value
+        return {lib().dmx_kernel_name(k).decode(): float(ms[k]) for k in range(ngroups)}
 
     def close(self):
         if getattr(self, "h", None):
