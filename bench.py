@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a plain 1 GiB device copy (torch) to calibrate HBM byte counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs one domain on one GPU; --gpus N>1 is not "
@@ -117,6 +119,14 @@ def main():
         sys.exit(2)
     from dmx import problems as P
     n = a.n
+    if a.calibrate:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 3 x 1 GiB read + 1 GiB write")
     log(f"[bench] building Orszag-Tang {n}^3 initial state")
     pb = P.orszag_tang(n)
     cells = n ** 3

@@ -24,7 +24,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Wno-unused-result", "-I", os.path.join(PKG, "..", "include")]
 UNITS = [
     ("dmx_f64.hip", ["-ffp-contract=off"]),
-    ("dmx_f32.hip", ["-ffp-contract=fast"]),
+    ("dmx_f32.hip", ["-ffp-contract=fast", "-ffast-math", "-fno-finite-math-only"]),
     ("dmx_abi.cpp", []),
 ]
 

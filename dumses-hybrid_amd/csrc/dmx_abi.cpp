@@ -146,7 +146,13 @@ int dmx_history(dmx_ctx* c, double out[9])
 {
     CTX_CHECK(c);
     if (!out) { dmx::set_error("null out"); return DMX_E_PARAM; }
-    return c->e->history(out);
+    return c->e->history(0, out);
+}
+int dmx_history_shear(dmx_ctx* c, double out[9])
+{
+    CTX_CHECK(c);
+    if (!out) { dmx::set_error("null out"); return DMX_E_PARAM; }
+    return c->e->history(1, out);
 }
 int dmx_last_advance_ms(dmx_ctx* c, double* ms)
 {

@@ -22,7 +22,7 @@ public:
     virtual int fargo(double dt) = 0;
     virtual int boundary(double time, double dt) = 0;
     virtual int advance(int nsteps, double* time, double* dts) = 0;
-    virtual int history(double out[9]) = 0;
+    virtual int history(int variant, double out[9]) = 0;
     virtual int last_advance_ms(double* ms) = 0;
     virtual int profile(int nsteps, double* time, double* ms, int nk) = 0;
 };

@@ -50,6 +50,7 @@ def params_to_c(p: Params) -> DmxParams:
 # every entry point include/dmx.h declares
 EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compute_dt",
            "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_advance", "dmx_history",
+           "dmx_history_shear",
            "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_last_error",
            "dmx_version")
 
@@ -82,6 +83,7 @@ def lib(path: str | None = None):
     L.dmx_boundary.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.dmx_advance.argtypes = [vp, ctypes.c_int32, dp, dp]
     L.dmx_history.argtypes = [vp, dp]
+    L.dmx_history_shear.argtypes = [vp, dp]
     L.dmx_last_advance_ms.argtypes = [vp, dp]
     L.dmx_profile.argtypes = [vp, ctypes.c_int32, dp, dp, ctypes.c_int32]
     L.dmx_kernel_name.argtypes = [ctypes.c_int32]
@@ -151,9 +153,10 @@ class Domain:
         self.time = t.value
         return t.value, dts[:nsteps]
 
-    def history(self) -> np.ndarray:
+    def history(self, shear: bool = False) -> np.ndarray:
         out = np.zeros(9)
-        _check(lib().dmx_history(self.h, _ptr(out)), "dmx_history")
+        fn = lib().dmx_history_shear if shear else lib().dmx_history
+        _check(fn(self.h, _ptr(out)), "dmx_history")
         return out
 
     def last_advance_ms(self) -> float:

@@ -242,6 +242,33 @@ def shearing_box(nx: int = 128, ny: int = 256, nz: int = 128, seed: int = 202612
                    "shearing_box", {"seed": seed})
 
 
+def mri_test(nx: int = 32, ny: int = 64, nz: int = 32, **kw) -> Problem:
+    """Isothermal shearing-box MRI of the reference's own test suite
+    (utils/test/test.py:975-990: problem mri, type 'Test', amp = 0, ISO=1):
+    box [-0.5,0.5]x[0,4]x[-0.5,0.5], Omega0 = ciso = 1e-3, beta = 1000,
+    gamma = 1.001 (utils/test/test.py:112), courant 0.8, shearing-periodic x;
+    rho = 1, rho*vx = 1e-5 cos(20 pi z) cos(10 pi x), Bz_L = Bz_R =
+    B0 sin(2 pi x) with B0 = sqrt(2 p0 / beta), p0 = ciso^2
+    (src/problem/mri/condinit.f90:38-108)."""
+    p = Params(nx=nx, ny=ny, nz=nz, xmin=-0.5, xmax=0.5, ymin=0.0, ymax=4.0, zmin=-0.5,
+               zmax=0.5, gamma=1.001, ciso=1e-3, omega0=1e-3, courant=0.8, iso=1,
+               bdtype=(SHEARINGBOX, PERIODIC, PERIODIC)).replace(**kw)
+    tp = 2.0 * np.pi
+    p0 = 1.0 * p.ciso * p.ciso
+    b0 = np.sqrt(2.0 * p0 / 1000.0)
+    xc = centers(p.xmin, p.dx, p.nx)
+    zc = centers(p.zmin, p.dz, p.nz)
+    u = np.zeros((11, p.nz + 6, p.ny + 6, p.nx + 6))
+    Z, X = zc[:, None, None], xc[None, None, :]
+    u[0] = 1.0
+    u[1] = 1.0e-2 * 1.0e-3 * np.cos(tp * 10.0 * Z) * np.cos(tp * 5.0 * X) + 0.0 * u[1]
+    bz = b0 * np.sin(tp * X) + 0.0 * u[7]
+    u[7] = bz
+    u[10] = bz
+    # condinit sets every cell, ghosts included; init() then applies boundary
+    return Problem(p, fill_periodic(u), "mri_test")
+
+
 def random_smooth(nx: int, ny: int, nz: int, seed: int = 1, kmax: int = 2,
                   mach: float = 1.0, beta: float = 1.0, b0=(0.3, 0.2, 0.1), **kw) -> Problem:
     """Random smooth periodic state for parity tests: a few Fourier modes of

@@ -121,6 +121,12 @@ int  dmx_advance(dmx_ctx* ctx, int32_t nsteps, double* time, double* dts);
  * magp, meanBx, meanBy, meanBz, divB. */
 int  dmx_history(dmx_ctx* ctx, double out[9]);
 
+/* The same sums with the shearing-box stresses that problem plugins such as
+ * the MRI setup substitute for the history routine
+ * (src/problem/mri/history.f90:54-90): maxwell = -<Bx By>,
+ * reynolds = <rho dvx dvy> (velocity fluctuations about the yz means). */
+int  dmx_history_shear(dmx_ctx* ctx, double out[9]);
+
 /* Synchronise and time: milliseconds of the last dmx_advance on the device. */
 int  dmx_last_advance_ms(dmx_ctx* ctx, double* ms);
 

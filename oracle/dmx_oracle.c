@@ -1458,9 +1458,15 @@ double dmxo_advance(dmxo* s, int nsteps, double time, double* dts)
     return time;
 }
 
-/* history sums, history.f90:62-118 (single rank) */
-void dmxo_history(dmxo* s, double out[9])
+/* history sums, history.f90:62-118 (single rank); shear = 1 gives the
+ * shearing-box stresses of the MRI problem's history (Maxwell -<Bx By>,
+ * Reynolds <rho dvx dvy>, src/problem/mri/history.f90:54-90) */
+void dmxo_history_variant(dmxo* s, int shear, double out[9]);
+void dmxo_history(dmxo* s, double out[9]) { dmxo_history_variant(s, 0, out); }
+
+void dmxo_history_variant(dmxo* s, int shear, double out[9])
 {
+    const int IT = shear ? IV : IW;   /* second velocity of the Reynolds stress */
     const dmx_params* p = &s->p;
     int nx = s->nx, ny = s->ny, nz = s->nz;
     double dtau = s->dx * s->dy * s->dz / (p->xmax - p->xmin) / (p->ymax - p->ymin) / (p->zmax - p->zmin);
@@ -1472,7 +1478,7 @@ void dmxo_history(dmxo* s, double out[9])
         for (int k = 1; k <= nz; k++)
             for (int j = 1; j <= ny; j++) {
                 su = su + U(i, j, k, IU) / U(i, j, k, IR);
-                sw = sw + U(i, j, k, IW) / U(i, j, k, IR);
+                sw = sw + U(i, j, k, IT) / U(i, j, k, IR);
             }
         mu[i + 2] = su / (double)ny / (double)nz;
         mw[i + 2] = sw / (double)ny / (double)nz;
@@ -1486,12 +1492,12 @@ void dmxo_history(dmxo* s, double out[9])
                 double bz = U(i, j, k, IC) + U(i, j, k + 1, IC);
                 mass = mass + U(i, j, k, IR);
                 magp = ((magp + forth * (bx * bx)) + forth * (by * by)) + forth * (bz * bz);
-                maxwell = maxwell - forth * bx * bz;
+                maxwell = maxwell - forth * bx * (shear ? by : bz);
                 mb[0] = mb[0] + U(i, j, k, IA);
                 mb[1] = mb[1] + U(i, j, k, IB);
                 mb[2] = mb[2] + U(i, j, k, IC);
                 reynolds = reynolds + U(i, j, k, IR) * dtau * (U(i, j, k, IU) / U(i, j, k, IR) - mu[i + 2])
-                                      * (U(i, j, k, IW) / U(i, j, k, IR) - mw[i + 2]);
+                                      * (U(i, j, k, IT) / U(i, j, k, IR) - mw[i + 2]);
                 divB = ((divB + (U(i + 1, j, k, IA) - U(i, j, k, IA)) / s->dx)
                         + (U(i, j + 1, k, IB) - U(i, j, k, IB)) / s->dy)
                        + (U(i, j, k + 1, IC) - U(i, j, k, IC)) / s->dz;

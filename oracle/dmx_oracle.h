@@ -33,6 +33,8 @@ void   dmxo_boundary(dmxo* s, double time, double dt); /* boundary.f90:18 */
 /* nsteps of the reference loop (dumses.f90:80-120); returns the new time */
 double dmxo_advance(dmxo* s, int nsteps, double time, double* dts);
 void   dmxo_history(dmxo* s, double out[9]);     /* history.f90:62-118   */
+/* shear = 1: stresses of src/problem/mri/history.f90 (-<BxBy>, <rho dvx dvy>) */
+void   dmxo_history_variant(dmxo* s, int shear, double out[9]);
 
 /* Function-level entry points of the Riemann solvers.
  * 1D: q = [rho, p, u_n, B_n, u_t1, B_t1, u_t2, B_t2] left and right;

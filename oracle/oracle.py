@@ -54,6 +54,7 @@ def lib():
         L.dmxo_advance.restype = ctypes.c_double
         L.dmxo_advance.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp]
         L.dmxo_history.argtypes = [ctypes.c_void_p, dp]
+        L.dmxo_history_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
         L.dmxo_riemann1d.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, dp, dp]
         L.dmxo_riemann2d.restype = ctypes.c_double
@@ -104,9 +105,9 @@ class Oracle:
         t = lib().dmxo_advance(self.h, nsteps, time, _ptr(dts))
         return t, dts[:nsteps]
 
-    def history(self) -> np.ndarray:
+    def history(self, shear: bool = False) -> np.ndarray:
         out = np.zeros(9)
-        lib().dmxo_history(self.h, _ptr(out))
+        lib().dmxo_history_variant(self.h, 1 if shear else 0, _ptr(out))
         return out
 
     def close(self):

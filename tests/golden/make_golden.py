@@ -162,7 +162,21 @@ def write_kernels():
     print("riemann_kernels: %d states" % n)
 
 
+def copy_mri_history():
+    """The reference's own numeric golden file: the history of its isothermal
+    shearing-box MRI test (utils/test/fig/reference/shearing/mri/history.txt,
+    written by src/subroutines/history.f90:148-165), kept as data."""
+    src = "/root/reference/utils/test/fig/reference/shearing/mri/history.txt"
+    rows = [ln for ln in open(src) if ln.strip() and not ln.lstrip().startswith("#")]
+    vals = np.array([[float(x) for x in ln.split()] for ln in rows])
+    np.savez_compressed(os.path.join(HERE, "mri_history_reference.npz"), history=vals,
+                        columns=np.array(["time", "dt", "mass", "maxwell", "reynolds", "max+rey",
+                                          "magp", "meanBx", "meanBy", "meanBz", "divB"]))
+    print(f"mri_history_reference: {vals.shape[0]} rows")
+
+
 def main():
+    copy_mri_history()
     if not (O.reference_available(0) and O.reference_available(1)):
         raise SystemExit("build the reference first: make -f oracle/ref.mk")
     write_kernels()
