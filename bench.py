@@ -31,40 +31,73 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# algorithmic bytes per cell of each kernel group (inputs read once, outputs
-# written once; 8-byte words in float64, T-size words for the bulk in float32)
+# algorithmic bytes per interior cell of each kernel group: its inputs read
+# once and its outputs written once (8-byte words in float64; T-size words
+# for the bulk in float32, face B always float64).  The trace summary has 47
+# words per cell, the face fluxes 18, the edge EMFs 3.
 def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
     hydro, bfield = 5 * wt, 3 * wb
     table = {
         "compute_dt": hydro + bfield,
         "trace": hydro + bfield + 47 * wt,
-        "faces": 2 * 47 * wt // 2 + 3 * 6 * wt,        # summary read, 3 faces x 6 words
-        "edges": 47 * wt + 3 * wt,                      # summary read, 3 EMFs written
-        "update_ct": 2 * (hydro + bfield) + 3 * 6 * wt + 3 * wt,
+        "faces": 47 * wt + 18 * wt,
+        "edges": 47 * wt + 3 * wt,
+        "update_ct": 2 * (hydro + bfield) + 18 * wt + 3 * wt,
         "boundary": 0.0,
     }
     return float(table.get(group, 0.0))
 
 
-def run_gpu(pb, steps, warmup, precision, profile_steps):
+# bytes leaving L2 per interior cell and launch, (2 x FETCH_SIZE + WRITE_SIZE)
+# x 1 KiB from the two rocprofv3 counter runs at 256^3 summarised in
+# profiles/mhd_step_ot256_hbm_bytes.md (FETCH_SIZE scaled by 2.00, WRITE_SIZE
+# by 1.00 against the known-size calibration kernels of the same runs)
+MEASURED_TRAFFIC = {
+    "f64": {"trace": 757.1, "faces": 654.9, "edges": 834.0, "update_ct": 417.0,
+            "compute_dt": 85.4, "boundary": 17.5},
+    "f32": {"trace": 486.0, "faces": 423.3, "edges": 567.5, "update_ct": 279.9,
+            "compute_dt": 64.6, "boundary": 15.1},
+}
+
+
+def roofline(prof: dict, cells: int, prec: str) -> dict | None:
+    """HBM roofline of the dominant kernel group of a profiled run."""
+    if not prof:
+        return None
+    dom = max(prof, key=prof.get)
+    ms = prof[dom]
+    bpc = algo_bytes(dom, 8 if prec == "f64" else 4)
+    ach = bpc * cells / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    meas = MEASURED_TRAFFIC.get(prec, {}).get(dom)
+    traffic = meas * cells if meas is not None else None
+    return dict(bound="hbm", kernel=dom, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=ach / HBM_PEAK_GBS, traffic=traffic, algo_bytes_per_cell=bpc,
+                traffic_gbs=(traffic / (ms * 1e-3) / 1e9 if traffic and ms > 0 else None),
+                kernel_ms=ms, step_kernel_ms=prof)
+
+
+def run_gpu(pb, steps, warmup, precision):
+    """Warm-up, then the timed region: `steps` steps of the instrumented
+    advance (HIP events around every kernel group on the library's stream,
+    read once after the run), bracketed by device synchronisation."""
     from dmx import abi
     p = pb.params.replace(precision=precision)
     d = abi.Domain(p, pb.uin)
-    t, _ = d.advance(warmup, 0.0)
+    t, _ = d.advance(warmup, 0.0)          # ends with a stream synchronisation
     t0 = time.perf_counter()
-    t, dts = d.advance(steps, t)
+    prof = d.profile(steps, t)             # ends with a stream synchronisation
     wall = time.perf_counter() - t0
     dev_ms = d.last_advance_ms()
-    prof = d.profile(profile_steps, t) if profile_steps else {}
     state = d.download()
     d.close()
-    return dict(wall=wall, dev_ms=dev_ms, dts=dts, prof=prof, state=state)
+    return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state)
 
 
-def cpu_baseline(n=64, steps=6):
-    """The compiled reference (serial OpenMP build timed with 1 thread, its
+def cpu_baseline(n=128, steps=3):
+    """The compiled reference (its OpenMP -O3 build, run with 1 thread, its
     fastest setting) on a bounded sample of the same workload: Orszag-Tang
-    at n^3 for a few steps.  Falls back to the C restatement."""
+    at n^3 for a few steps, timed by the harness around its step loop only
+    (no initialisation or dump I/O).  Falls back to the C restatement."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     from dmx import problems as P
@@ -85,9 +118,8 @@ def cpu_baseline(n=64, steps=6):
                                                    (p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax),
                                                    p.gamma, p.ciso, pb.uin))
             env = dict(os.environ, OMP_NUM_THREADS="1")
-            t0 = time.perf_counter()
             subprocess.run([exe], cwd=work, env=env, check=True, capture_output=True)
-            wall = time.perf_counter() - t0
+            wall = float(open(os.path.join(work, "loop_seconds.txt")).read())
         finally:
             shutil.rmtree(work, ignore_errors=True)
         kind = "reference"
@@ -98,7 +130,9 @@ def cpu_baseline(n=64, steps=6):
         wall = time.perf_counter() - t0
         kind = "port"
     return dict(value=n ** 3 * steps / wall, unit="zone-cycles/s", cores=1, kind=kind,
-                sample=f"Orszag-Tang {n}^3, {steps} steps incl. init I/O, 1 thread")
+                sample=f"Orszag-Tang {n}^3 (same initial field at 1/8 of the cells), {steps} steps "
+                       f"(compute_dt+godunov+boundary), step loop only, OMP_NUM_THREADS=1, "
+                       f"{wall:.1f} s")
 
 
 def main():
@@ -107,7 +141,6 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -121,23 +154,26 @@ def main():
     n = a.n
     if a.calibrate:
         import torch
+        # 1 GiB written by a fill kernel, then 1 GiB read + 1 GiB written
+        # by an elementwise kernel: the known sizes against which the HBM
+        # byte counters (FETCH_SIZE, WRITE_SIZE) are scaled
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+        torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
-        log("[bench] calibration copy: 3 x 1 GiB read + 1 GiB write")
+        del src, dst
+        log("[bench] calibration: 1 GiB fill, 1 GiB read + 1 GiB write elementwise")
     log(f"[bench] building Orszag-Tang {n}^3 initial state")
     pb = P.orszag_tang(n)
     cells = n ** 3
     log("[bench] float64 run")
-    r64 = run_gpu(pb, a.steps, a.warmup, 0, a.profile_steps)
+    r64 = run_gpu(pb, a.steps, a.warmup, 0)
     v64 = cells * a.steps / r64["wall"]
     log(f"[bench] float64: {v64:.4g} zone-cycles/s, {1e3 * r64['wall'] / a.steps:.3f} ms/step")
     out = {}
     if not a.no_fp32:
         log("[bench] float32-bulk run")
-        r32 = run_gpu(pb, a.steps, a.warmup, 1, a.profile_steps)
+        r32 = run_gpu(pb, a.steps, a.warmup, 1)
         v32 = cells * a.steps / r32["wall"]
         u64, u32 = P.interior(r64["state"]), P.interior(r32["state"])
         names = ["rho", "rhou", "rhov", "rhow", "E", "Bx", "By", "Bz"]
@@ -148,19 +184,12 @@ def main():
         norms = [np.abs(u64[0]).sum(), mom, mom, mom, np.abs(u64[4]).sum(), mag, mag, mag]
         l1 = {names[v]: float(np.abs(u32[v] - u64[v]).sum() / norms[v]) for v in range(8)}
         out["fp32"] = dict(value=v32, ms_per_step=1e3 * r32["wall"] / a.steps, speedup_vs_fp64=v32 / v64,
-                           l1_rel_vs_fp64=l1, kernel_ms=r32["prof"],
+                           l1_rel_vs_fp64=l1,
                            dtype="f32 bulk, f64 face-B/CT/pressure recovery/sums")
         log(f"[bench] float32: {v32:.4g} zone-cycles/s ({v32 / v64:.2f}x), L1 {l1}")
-    prof = r64["prof"]
-    dom = max(prof, key=prof.get) if prof else None
-    roof = None
-    if dom:
-        ms = prof[dom]
-        bpc = algo_bytes(dom, 8)
-        ach = bpc * cells / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-        roof = dict(bound="hbm", kernel=dom, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=ach / HBM_PEAK_GBS, traffic=None, algo_bytes_per_cell=bpc,
-                    kernel_ms=ms, step_kernel_ms=prof)
+    roof = roofline(r64["prof"], cells, "f64")
+    if "fp32" in out:
+        out["fp32"]["roofline"] = roofline(r32["prof"], cells, "f32")
     res = {
         "metric": METRIC, "value": v64, "unit": "zone-cycles/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * r64["wall"] / a.steps,
@@ -170,6 +199,10 @@ def main():
                    "cells": cells, "parallelism": "single domain, 1 GPU"},
         "device_ms_per_step": r64["dev_ms"] / a.steps,
         "roofline": roof,
+        # the whole step against the compulsory state traffic: one read and one
+        # write of the 8 stored words of every cell, 128 B per zone-cycle
+        "step_roofline": {"bytes_per_zone_cycle": 128, "achieved": 128 * v64 / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": 128 * v64 / 1e9 / HBM_PEAK_GBS},
     }
     res.update(out)
     if not a.no_cpu:

@@ -130,11 +130,13 @@ int  dmx_history_shear(dmx_ctx* ctx, double out[9]);
 /* Synchronise and time: milliseconds of the last dmx_advance on the device. */
 int  dmx_last_advance_ms(dmx_ctx* ctx, double* ms);
 
-/* Instrumented dmx_advance: runs nsteps with HIP events around every kernel
- * group on the context's stream and returns in ms[id] the mean device time
- * per step of each group id < nkernels (names from dmx_kernel_name; groups
- * that did not run report 0).  No counterpart in the reference, whose timers
- * come from a source preprocessor (utils/pyutils/preproc/preprocess.py). */
+/* Instrumented dmx_advance: runs nsteps (<= 100000) with HIP events around
+ * every kernel group of every step on the context's stream, synchronises
+ * once at the end, and returns in ms[id] the mean device time per step of
+ * each group id < nkernels (names from dmx_kernel_name; groups that did not
+ * run report 0); dmx_last_advance_ms then gives the whole run.  No
+ * counterpart in the reference, whose timers come from a source
+ * preprocessor (utils/pyutils/preproc/preprocess.py). */
 int  dmx_profile(dmx_ctx* ctx, int32_t nsteps, double* time, double* ms, int32_t nkernels);
 const char* dmx_kernel_name(int32_t id);
 

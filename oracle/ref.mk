@@ -8,7 +8,7 @@
 # Targets (all MPI=0, NDIM=3, Cartesian; macros as in src/Makefile.am:30):
 #   _ref/iso0/ref_harness      adiabatic, serial     (parity checker)
 #   _ref/iso1/ref_harness      isothermal, serial    (parity checker)
-#   _ref/iso0_omp/ref_harness  adiabatic, OpenMP     (timed CPU baseline)
+#   _ref/iso0_omp/ref_harness  adiabatic, OpenMP -O3 (timed CPU baseline)
 # The serial builds are the checker because the reference's OpenMP build
 # reads PRIVATE copies of module scratch that the Riemann solvers never
 # write (src/modules/solvers/solver.f90:127-128 with :209-242,
@@ -49,7 +49,7 @@ $(OUT)/iso0/ref_harness: $(SRCS)
 $(OUT)/iso1/ref_harness: $(SRCS)
 	$(call build,$(OUT)/iso1,1,)
 $(OUT)/iso0_omp/ref_harness: $(SRCS)
-	$(call build,$(OUT)/iso0_omp,0,-fopenmp)
+	$(call build,$(OUT)/iso0_omp,0,-O3 -fopenmp)
 
 clean:
 	rm -rf $(OUT)

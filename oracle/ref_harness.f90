@@ -23,6 +23,7 @@ program ref_harness
   implicit none
 
   integer :: nsteps, dumpevery, istep, u
+  integer(kind=8) :: c0, c1, crate
   character(len=16) :: mode
   namelist /harness_params/ nsteps, dumpevery, mode
 
@@ -45,6 +46,7 @@ program ref_harness
 
   open(unit=21, file='dt.raw', access='stream', form='unformatted', &
        status='replace')
+  call system_clock(count=c0, count_rate=crate)
   do istep = 1, nsteps
      call compute_dt(dt)
      call godunov
@@ -56,7 +58,12 @@ program ref_harness
         if (mod(istep, dumpevery) == 0) call dump_state(istep)
      endif
   end do
+  call system_clock(count=c1)
   close(21)
+  ! wall time of the step loop alone (no initialisation or dump I/O)
+  open(newunit=u, file='loop_seconds.txt', status='replace')
+  write(u, '(ES24.16)') real(c1 - c0, kind=8) / real(crate, kind=8)
+  close(u)
   call dump_state(nsteps)
 
 contains
