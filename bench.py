@@ -42,6 +42,7 @@ def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
         "trace": hydro + bfield + 47 * wt,
         "faces": 47 * wt + 18 * wt,
         "edges": 47 * wt + 3 * wt,
+        "flux": 47 * wt + 18 * wt + 3 * wt,           # faces and edges in one pass
         "update_ct": 2 * (hydro + bfield) + 18 * wt + 3 * wt,
         "boundary": 0.0,
     }

@@ -29,22 +29,24 @@ UNITS = [
 ]
 
 
-def _compile(unit):
+def _compile(unit, out=OUT, defines=()):
     src, extra = unit
-    obj = os.path.join(OUT, src.rsplit(".", 1)[0] + ".o")
+    obj = os.path.join(out, src.rsplit(".", 1)[0] + ".o")
     lang = ["-x", "hip"] if src.endswith(".cpp") else []
-    cmd = [HIPCC, *COMMON, *extra, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *COMMON, *extra, *defines, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OUT, exist_ok=True)
+def build(verbose: bool = False, out: str = OUT, defines=()) -> str:
+    """Compiles and links libdmx.so into `out`; `defines` are extra -D
+    options for the kernel sources (tuning variants)."""
+    os.makedirs(out, exist_ok=True)
     with ThreadPoolExecutor(len(UNITS)) as ex:
-        objs = list(ex.map(_compile, UNITS))
-    lib = os.path.join(OUT, "libdmx.so")
+        objs = list(ex.map(lambda u: _compile(u, out, tuple(defines)), UNITS))
+    lib = os.path.join(out, "libdmx.so")
     tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -65,6 +65,16 @@ static int validate(const dmx_params* p)
         set_error("unknown precision %d", p->precision);
         return DMX_E_PARAM;
     }
+    {
+        // the kernels address one plane of float64 with 32-bit byte offsets
+        unsigned long long px = ((unsigned long long)p->nx + 6 + 15) / 16 * 16;
+        unsigned long long plane = px * (unsigned long long)(p->ny + 6) * (unsigned long long)(p->nz + 6) * 8ull;
+        if (plane >= (1ull << 32)) {
+            set_error("domain too large: one float64 plane of (nx+6)(ny+6)(nz+6) cells must stay below 4 GiB "
+                      "(about 800^3 cells)");
+            return DMX_E_PARAM;
+        }
+    }
     return DMX_OK;
 }
 

@@ -21,6 +21,10 @@ __device__ __forceinline__ double smallr_d() { return 1e-10; }
 
 template <typename T> __device__ __forceinline__ T fmx(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T fmn(T a, T b) { return a < b ? a : b; }
+// float bulk (no bit-exactness to keep): one v_max_f32 / v_min_f32; differs
+// from the comparison form only for NaN and the sign of a zero result
+template <> __device__ __forceinline__ float fmx<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+template <> __device__ __forceinline__ float fmn<float>(float a, float b) { return __builtin_fminf(a, b); }
 template <typename T> __device__ __forceinline__ T sgn1(T x) { return copysign(T(1), x); }
 template <typename T> __device__ __forceinline__ T fab(T x) { return fabs(x); }
 
