@@ -2,8 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <string.h>
+
+#include <vector>
 
 #include "dmx_engine.h"
 
@@ -20,6 +23,7 @@ void set_error(const char* fmt, ...)
 
 struct dmx_ctx {
     dmx::Engine* e;
+    dmx_params p;
 };
 
 static int validate(const dmx_params* p)
@@ -93,7 +97,7 @@ int dmx_create(const dmx_params* p, dmx_ctx** ctx)
     }
     dmx::Engine* e = p->precision == DMX_FP32 ? dmx::make_engine_f32(*p, &rc) : dmx::make_engine_f64(*p, &rc);
     if (!e) return rc ? rc : DMX_E_HIP;
-    *ctx = new dmx_ctx{e};
+    *ctx = new dmx_ctx{e, *p};
     return DMX_OK;
 }
 
@@ -176,6 +180,98 @@ int dmx_profile(dmx_ctx* c, int32_t nsteps, double* time, double* ms, int32_t nk
     if (!time || !ms || nkernels < 0) { dmx::set_error("bad profile arguments"); return DMX_E_PARAM; }
     return c->e->profile(nsteps, time, ms, nkernels);
 }
+// ---------------------------------------------------------------------------
+// The reference's binary dump (writer src/subroutines/output.f90:224-235,
+// reader src/subroutines/restart.f90:212-223): a Fortran unformatted
+// sequential file of five records, each framed by 4-byte little-endian
+// length markers:
+//   int32[6] nx, ny, nz, nxslice, nyslice, nzslice
+//   f64[5]   time, dt, dx, dy, dz
+//   f64[6]   xmin, xmax, ymin, ymax, zmin, zmax
+//   f64[2]   gamma, ciso
+//   f64      uin(-2:nx+3, -2:ny+3, -2:nz+3, 1:11)
+// ---------------------------------------------------------------------------
+static bool put_record(FILE* f, const void* data, size_t n)
+{
+    int32_t m = (int32_t)n;
+    return fwrite(&m, 4, 1, f) == 1 && (n == 0 || fwrite(data, 1, n, f) == n) && fwrite(&m, 4, 1, f) == 1;
+}
+
+static bool get_record(FILE* f, void* data, size_t n)
+{
+    int32_t m0 = 0, m1 = 0;
+    if (fread(&m0, 4, 1, f) != 1 || m0 < 0 || (size_t)m0 != n) return false;
+    if (n && fread(data, 1, n, f) != n) return false;
+    return fread(&m1, 4, 1, f) == 1 && m1 == m0;
+}
+
+static size_t uin_count(const dmx_params& p)
+{
+    return (size_t)(p.nx + 6) * (size_t)(p.ny + 6) * (size_t)(p.nz + 6) * 11;
+}
+
+int dmx_write_dump(dmx_ctx* c, const char* path, double time, double dt)
+{
+    CTX_CHECK(c);
+    if (!path) { dmx::set_error("null path"); return DMX_E_PARAM; }
+    const dmx_params& p = c->p;
+    const size_t n = uin_count(p);
+    if (n * 8 > 0x7fffffffull) {
+        dmx::set_error("uin record of %zu bytes exceeds the 2 GiB record of the binary format", n * 8);
+        return DMX_E_PARAM;
+    }
+    std::vector<double> u(n);
+    int rc = c->e->download(u.data());
+    if (rc) return rc;
+    FILE* f = fopen(path, "wb");
+    if (!f) { dmx::set_error("cannot open %s for writing", path); return DMX_E_PARAM; }
+    const int32_t hdr[6] = {p.nx, p.ny, p.nz, 1, 1, 1};
+    const double r2[5] = {time, dt, (p.xmax - p.xmin) / p.nx, (p.ymax - p.ymin) / p.ny, (p.zmax - p.zmin) / p.nz};
+    const double r3[6] = {p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax};
+    const double r4[2] = {p.gamma, p.ciso};
+    bool ok = put_record(f, hdr, sizeof(hdr)) && put_record(f, r2, sizeof(r2)) && put_record(f, r3, sizeof(r3))
+              && put_record(f, r4, sizeof(r4)) && put_record(f, u.data(), n * 8);
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) { dmx::set_error("write error on %s", path); return DMX_E_PARAM; }
+    return DMX_OK;
+}
+
+int dmx_read_dump(dmx_ctx* c, const char* path, double* time, double* dt)
+{
+    CTX_CHECK(c);
+    if (!path || !time || !dt) { dmx::set_error("null argument"); return DMX_E_PARAM; }
+    const dmx_params& p = c->p;
+    FILE* f = fopen(path, "rb");
+    if (!f) { dmx::set_error("cannot open %s", path); return DMX_E_PARAM; }
+    int32_t hdr[6];
+    double r2[5], r3[6], r4[2];
+    bool ok = get_record(f, hdr, sizeof(hdr)) && get_record(f, r2, sizeof(r2)) && get_record(f, r3, sizeof(r3))
+              && get_record(f, r4, sizeof(r4));
+    if (!ok) { fclose(f); dmx::set_error("%s: not a binary dump (bad record framing)", path); return DMX_E_PARAM; }
+    if (hdr[0] != p.nx || hdr[1] != p.ny || hdr[2] != p.nz || hdr[3] != 1 || hdr[4] != 1 || hdr[5] != 1) {
+        fclose(f);
+        dmx::set_error("%s: dump of %dx%dx%d on %dx%dx%d ranks does not match the %dx%dx%d context", path, hdr[0],
+                       hdr[1], hdr[2], hdr[3], hdr[4], hdr[5], p.nx, p.ny, p.nz);
+        return DMX_E_PARAM;
+    }
+    const double want3[6] = {p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax};
+    if (memcmp(r3, want3, sizeof(r3)) != 0 || r4[0] != p.gamma || r4[1] != p.ciso) {
+        fclose(f);
+        dmx::set_error("%s: domain bounds, gamma or ciso differ from the context's", path);
+        return DMX_E_PARAM;
+    }
+    const size_t n = uin_count(p);
+    std::vector<double> u(n);
+    ok = get_record(f, u.data(), n * 8);
+    fclose(f);
+    if (!ok) { dmx::set_error("%s: truncated or mismatched uin record", path); return DMX_E_PARAM; }
+    int rc = c->e->upload(u.data());
+    if (rc) return rc;
+    *time = r2[0];
+    *dt = r2[1];
+    return DMX_OK;
+}
+
 const char* dmx_kernel_name(int32_t id)
 {
     return (id >= 0 && id < dmx::KG_N) ? dmx::kKernelGroupNames[id] : "";

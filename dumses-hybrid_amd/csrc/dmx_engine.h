@@ -8,7 +8,7 @@
 
 #include <stdint.h>
 
-#include "../../include/dmx.h"
+#include "dmx.h"
 
 namespace dmx {
 

@@ -50,7 +50,7 @@ def params_to_c(p: Params) -> DmxParams:
 # every entry point include/dmx.h declares
 EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compute_dt",
            "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_advance", "dmx_history",
-           "dmx_history_shear",
+           "dmx_history_shear", "dmx_write_dump", "dmx_read_dump",
            "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_last_error",
            "dmx_version")
 
@@ -85,6 +85,8 @@ def lib(path: str | None = None):
     L.dmx_history.argtypes = [vp, dp]
     L.dmx_history_shear.argtypes = [vp, dp]
     L.dmx_last_advance_ms.argtypes = [vp, dp]
+    L.dmx_write_dump.argtypes = [vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double]
+    L.dmx_read_dump.argtypes = [vp, ctypes.c_char_p, dp, dp]
     L.dmx_profile.argtypes = [vp, ctypes.c_int32, dp, dp, ctypes.c_int32]
     L.dmx_kernel_name.argtypes = [ctypes.c_int32]
     L.dmx_kernel_name.restype = ctypes.c_char_p
@@ -152,6 +154,19 @@ class Domain:
                                  _ptr(dts) if want_dts else None), "dmx_advance")
         self.time = t.value
         return t.value, dts[:nsteps]
+
+    def write_dump(self, path: str, time: float, dt: float) -> None:
+        """The reference's binary dump of the current state (dmx_write_dump)."""
+        d = os.path.dirname(os.path.abspath(path))
+        os.makedirs(d, exist_ok=True)
+        _check(lib().dmx_write_dump(self.h, path.encode(), float(time), float(dt)), "dmx_write_dump")
+
+    def read_dump(self, path: str) -> tuple[float, float]:
+        """Restart from a binary dump (dmx_read_dump); returns (time, dt)."""
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        _check(lib().dmx_read_dump(self.h, path.encode(), ctypes.byref(t), ctypes.byref(dt)), "dmx_read_dump")
+        self.time = t.value
+        return t.value, dt.value
 
     def history(self, shear: bool = False) -> np.ndarray:
         out = np.zeros(9)

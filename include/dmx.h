@@ -127,6 +127,22 @@ int  dmx_history(dmx_ctx* ctx, double out[9]);
  * reynolds = <rho dvx dvy> (velocity fluctuations about the yz means). */
 int  dmx_history_shear(dmx_ctx* ctx, double out[9]);
 
+/* Writes the context's state as the reference's binary dump
+ * (src/subroutines/output.f90:224-235, io_type='binary'): five Fortran
+ * unformatted sequential records -- int32 nx,ny,nz,1,1,1; f64 time,dt,dx,dy,dz;
+ * f64 xmin..zmax; f64 gamma,ciso; f64 uin -- each framed by 4-byte
+ * little-endian length markers.  `path` is the full file name (the reference
+ * uses output_NNNNNN/data.000000.bin, src/modules/file.f90:21-51); its
+ * directory must exist.  The uin record must stay below 2 GiB (about 270^3). */
+int  dmx_write_dump(dmx_ctx* ctx, const char* path, double time, double dt);
+
+/* Restart: reads a binary dump written by the reference or dmx_write_dump
+ * (src/subroutines/restart.f90:212-223) into the context and returns its
+ * time and dt.  Sizes, single-rank slicing, domain bounds, gamma and ciso must
+ * match the context (the reference allocates before restart and would read
+ * a mismatched file out of bounds; here it is DMX_E_PARAM). */
+int  dmx_read_dump(dmx_ctx* ctx, const char* path, double* time, double* dt);
+
 /* Synchronise and time: milliseconds of the last dmx_advance on the device. */
 int  dmx_last_advance_ms(dmx_ctx* ctx, double* ms);
 

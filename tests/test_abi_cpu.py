@@ -53,6 +53,7 @@ def test_struct_layout_matches_header(tmp_path):
     dict(bdtype=(P.PERIODIC, P.SHEARINGBOX, P.PERIODIC)), dict(fargo=1, omega0=0.0),
     dict(gamma=1.0), dict(courant=0.0), dict(precision=2), dict(xmax=0.0),
     dict(bdtype=(P.SHEARINGBOX, P.ZERO_GRADIENT, P.PERIODIC), omega0=1.0),
+    dict(nx=900, ny=900, nz=900),        # a float64 plane above 4 GiB
 ])
 def test_invalid_parameters_rejected(bad):
     p = P.Params(nx=8, ny=8, nz=8).replace(**bad)
@@ -70,3 +71,6 @@ def test_null_context_is_an_error():
     dt = ctypes.c_double()
     assert lib.dmx_compute_dt(None, ctypes.byref(dt)) == -5
     assert b"context" in lib.dmx_last_error()
+    assert lib.dmx_write_dump(None, b"/nonexistent/x.bin", 0.0, 0.0) == -5
+    t = ctypes.c_double()
+    assert lib.dmx_read_dump(None, b"/nonexistent/x.bin", ctypes.byref(t), ctypes.byref(dt)) == -5

@@ -2,7 +2,8 @@
 (src/dumses.f90 with its namelist input, restart reader and binary output),
 linked against dmx_shim.f90 + libdmx.so instead of its physics files
 (dumses-hybrid_amd/fortran/dropin.mk), must write the same dump as the
-reference for the same run."""
+reference for the same run; the C ABI's restart reader (dmx_read_dump) must
+read that Fortran-written file back."""
 import os
 import shutil
 import subprocess
@@ -45,6 +46,13 @@ def run_dropin(p, uin, dts, nsteps, env_extra=None):
         assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
         d = read_dump(dump_path(2, 0, work))
         hist = open(os.path.join(work, "history.txt")).read()
+        # the same file through the C ABI's restart reader
+        from dmx import abi
+        dom = abi.Domain(p, uin)
+        t_abi, dt_abi = dom.read_dump(dump_path(2, 0, work))
+        assert (t_abi, dt_abi) == (d.time, d.dt)
+        assert np.array_equal(P.interior(dom.download()), P.interior(d.uin))
+        dom.close()
         return d, hist, r.stdout
     finally:
         shutil.rmtree(work, ignore_errors=True)
