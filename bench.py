@@ -54,10 +54,10 @@ def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
 # profiles/mhd_step_ot256_hbm_bytes.md (FETCH_SIZE scaled by 2.00, WRITE_SIZE
 # by 1.00 against the known-size calibration kernels of the same runs)
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 757.1, "faces": 654.9, "edges": 834.0, "update_ct": 417.0,
-            "compute_dt": 85.4, "boundary": 17.5},
-    "f32": {"trace": 486.0, "faces": 423.3, "edges": 567.5, "update_ct": 279.9,
-            "compute_dt": 64.6, "boundary": 15.1},
+    "f64": {"trace": 756.2, "flux": 1647.4, "update_ct": 416.3, "compute_dt": 86.0,
+            "boundary": 17.5},
+    "f32": {"trace": 479.4, "flux": 1031.0, "update_ct": 279.9, "compute_dt": 64.7,
+            "boundary": 15.1},
 }
 
 
@@ -142,6 +142,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--workload", choices=("ot", "shear"), default="ot",
+                    help="ot: 3D Orszag-Tang n^3 (the metric's workload); shear: the shearing-box "
+                         "MRI with FARGO at 128x256x128 (a parity workload, reported for reference)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -164,9 +167,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log("[bench] calibration: 1 GiB fill, 1 GiB read + 1 GiB write elementwise")
-    log(f"[bench] building Orszag-Tang {n}^3 initial state")
-    pb = P.orszag_tang(n)
-    cells = n ** 3
+    if a.workload == "shear":
+        log("[bench] building the shearing-box MRI 128x256x128 initial state")
+        pb = P.shearing_box(128, 256, 128, fargo=1)
+        workload = "shearing-box MRI 128x256x128, FARGO, HLLD/HLLD, MC, CFL 0.7"
+    else:
+        log(f"[bench] building Orszag-Tang {n}^3 initial state")
+        pb = P.orszag_tang(n)
+        workload = f"3D Orszag-Tang {n}^3 periodic, HLLD/HLLD, MC slopes, CFL 0.7"
+    cells = pb.params.nx * pb.params.ny * pb.params.nz
     log("[bench] float64 run")
     r64 = run_gpu(pb, a.steps, a.warmup, 0)
     v64 = cells * a.steps / r64["wall"]
@@ -196,8 +205,7 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * r64["wall"] / a.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3D Orszag-Tang {n}^3 periodic, HLLD/HLLD, MC slopes, CFL 0.7",
-                   "cells": cells, "parallelism": "single domain, 1 GPU"},
+        "config": {"workload": workload, "cells": cells, "parallelism": "single domain, 1 GPU"},
         "device_ms_per_step": r64["dev_ms"] / a.steps,
         "roofline": roof,
         # the whole step against the compulsory state traffic: one read and one

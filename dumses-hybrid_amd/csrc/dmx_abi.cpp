@@ -61,6 +61,7 @@ static int validate(const dmx_params* p)
     if (p->fargo && !(p->omega0 > 0.0)) { set_error("fargo needs omega0 > 0"); return DMX_E_PARAM; }
     if (!(p->gamma > 1.0)) { set_error("gamma must be > 1"); return DMX_E_PARAM; }
     if (!(p->courant > 0.0)) { set_error("courant must be > 0"); return DMX_E_PARAM; }
+    if (!(p->nu >= 0.0) || !(p->eta >= 0.0)) { set_error("nu and eta must be >= 0"); return DMX_E_PARAM; }
     if (!(p->xmax > p->xmin) || !(p->ymax > p->ymin) || !(p->zmax > p->zmin)) {
         set_error("empty domain");
         return DMX_E_PARAM;
@@ -149,6 +150,12 @@ int dmx_boundary(dmx_ctx* c, double time, double dt)
 {
     CTX_CHECK(c);
     return c->e->boundary(time, dt);
+}
+int dmx_dissipation(dmx_ctx* c, double time, double dt)
+{
+    CTX_CHECK(c);
+    if (!(dt > 0.0)) { dmx::set_error("dt must be > 0"); return DMX_E_PARAM; }
+    return c->e->dissipation(time, dt);
 }
 int dmx_advance(dmx_ctx* c, int32_t nsteps, double* time, double* dts)
 {

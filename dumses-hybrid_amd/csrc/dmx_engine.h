@@ -21,6 +21,7 @@ public:
     virtual int godunov(double time, double dt) = 0;
     virtual int fargo(double dt) = 0;
     virtual int boundary(double time, double dt) = 0;
+    virtual int dissipation(double time, double dt) = 0;
     virtual int advance(int nsteps, double* time, double* dts) = 0;
     virtual int history(int variant, double out[9]) = 0;
     virtual int last_advance_ms(double* ms) = 0;
@@ -29,9 +30,10 @@ public:
 
 // kernel groups timed by dmx_profile
 enum KernelGroup { KG_DT = 0, KG_TRACE, KG_FACES, KG_EDGES, KG_SHEARFLUX, KG_UPDATE, KG_FARGO, KG_BOUNDARY,
-                   KG_FLUX, KG_N };
+                   KG_FLUX, KG_DISSIPATION, KG_N };
 static const char* const kKernelGroupNames[KG_N] = {
-    "compute_dt", "trace", "faces", "edges", "shearing_flux", "update_ct", "fargo", "boundary", "flux"};
+    "compute_dt", "trace", "faces", "edges", "shearing_flux", "update_ct", "fargo", "boundary", "flux",
+    "dissipation"};
 
 // set by the engines, read by dmx_last_error()
 void set_error(const char* fmt, ...);

@@ -51,6 +51,7 @@ struct Geom {
 
 struct Phys {
     double gamma, ciso, omega0, courant;
+    double nu, eta;
     int iso, slope_type, riemann, riemann2d, fargo;
     int bc[3];
 };

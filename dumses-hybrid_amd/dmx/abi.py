@@ -31,6 +31,7 @@ class DmxParams(ctypes.Structure):
         ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
         ("gamma", ctypes.c_double), ("ciso", ctypes.c_double),
         ("omega0", ctypes.c_double), ("courant", ctypes.c_double),
+        ("nu", ctypes.c_double), ("eta", ctypes.c_double),
     ]
 
 
@@ -42,14 +43,14 @@ def params_to_c(p: Params) -> DmxParams:
     for i in range(3):
         c.bdtype[i] = int(p.bdtype[i])
     for k in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "gamma", "ciso", "omega0",
-              "courant"):
+              "courant", "nu", "eta"):
         setattr(c, k, float(getattr(p, k)))
     return c
 
 
 # every entry point include/dmx.h declares
 EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compute_dt",
-           "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_advance", "dmx_history",
+           "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_dissipation", "dmx_advance", "dmx_history",
            "dmx_history_shear", "dmx_write_dump", "dmx_read_dump",
            "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_last_error",
            "dmx_version")
@@ -81,6 +82,7 @@ def lib(path: str | None = None):
     L.dmx_godunov.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.dmx_fargo.argtypes = [vp, ctypes.c_double]
     L.dmx_boundary.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    L.dmx_dissipation.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.dmx_advance.argtypes = [vp, ctypes.c_int32, dp, dp]
     L.dmx_history.argtypes = [vp, dp]
     L.dmx_history_shear.argtypes = [vp, dp]
@@ -146,6 +148,9 @@ class Domain:
 
     def boundary(self, time: float, dt: float):
         _check(lib().dmx_boundary(self.h, time, dt), "dmx_boundary")
+
+    def dissipation(self, time: float, dt: float) -> None:
+        _check(lib().dmx_dissipation(self.h, time, dt), "dmx_dissipation")
 
     def advance(self, nsteps: int, time: float | None = None, want_dts: bool = True):
         t = ctypes.c_double(self.time if time is None else time)

@@ -50,6 +50,8 @@ class Params:
     fargo: int = 0
     iso: int = 0
     precision: int = 0  # 0 float64, 1 float32 bulk
+    nu: float = 0.0     # viscosity (model_params nu)
+    eta: float = 0.0    # resistivity (model_params eta)
 
     @property
     def dx(self):

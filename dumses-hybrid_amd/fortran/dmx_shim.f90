@@ -3,9 +3,11 @@
 !
 ! Links in place of the reference's external subroutines compute_dt
 ! (src/subroutines/physics.f90:18), godunov (src/subroutines/scheme.f90:104),
-! fargo_update (src/subroutines/fargo.f90:17) and boundary
-! (src/subroutines/boundary.f90:18), which the unchanged driver
-! (src/dumses.f90:82,88,97,101) and init (src/subroutines/init.f90:95) call.
+! fargo_update (src/subroutines/fargo.f90:17), boundary
+! (src/subroutines/boundary.f90:18) and dissipation
+! (src/subroutines/physics.f90:272), which the unchanged driver
+! (src/dumses.f90:82,88,97,101,104) and init (src/subroutines/init.f90:95)
+! call.
 ! All state stays in the reference's modules; the shim forwards to libdmx.so
 ! (include/dmx.h) and keeps uin on the device between host synchronisation
 ! points.  The host copy is refreshed where the OpenACC build of the
@@ -27,6 +29,7 @@ module dmx_bind
      integer(c_int32_t) :: fargo, precision
      real(c_double) :: xmin, xmax, ymin, ymax, zmin, zmax
      real(c_double) :: gamma, ciso, omega0, courant
+     real(c_double) :: nu, eta
   end type dmx_params_t
 
   interface
@@ -65,6 +68,11 @@ module dmx_bind
        type(c_ptr), value :: ctx
        real(c_double), value :: time, dt
      end function dmx_boundary
+     integer(c_int) function dmx_dissipation(ctx, time, dt) bind(C, name='dmx_dissipation')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: time, dt
+     end function dmx_dissipation
      type(c_ptr) function dmx_last_error() bind(C, name='dmx_last_error')
        import :: c_ptr
      end function dmx_last_error
@@ -128,6 +136,7 @@ contains
     p%xmin = xmin; p%xmax = xmax; p%ymin = ymin; p%ymax = ymax
     p%zmin = zmin; p%zmax = zmax
     p%gamma = gamma; p%ciso = ciso; p%omega0 = Omega0; p%courant = courant
+    p%nu = nu; p%eta = eta
     call dmx_check(dmx_create(p, ctx), 'dmx_create')
   end subroutine dmx_ensure_ctx
 
@@ -202,9 +211,12 @@ subroutine boundary
 end subroutine boundary
 
 subroutine dissipation
+  use dmx_bind
+  use variables, only: time, dt
   implicit none
-  print '(A)', 'dmx: viscosity/resistivity (nu, eta > 0) is not part of the device step'
-  error stop 1
+  call dmx_to_device()
+  call dmx_check(dmx_dissipation(ctx, time, dt), 'dmx_dissipation')
+  host_stale = .true.
 end subroutine dissipation
 
 !-------------------------------------------------------------------------------

@@ -82,6 +82,7 @@ typedef struct dmx_params {
     int32_t precision;           /* DMX_FP64 | DMX_FP32                       */
     double  xmin, xmax, ymin, ymax, zmin, zmax;
     double  gamma, ciso, omega0, courant;
+    double  nu, eta;             /* viscosity, resistivity (>= 0; 0 = off)    */
 } dmx_params;
 
 typedef struct dmx_ctx dmx_ctx;
@@ -110,9 +111,16 @@ int  dmx_fargo(dmx_ctx* ctx, double dt);
  * src/subroutines/boundary.f90:18-92). */
 int  dmx_boundary(dmx_ctx* ctx, double time, double dt);
 
+/* Dissipation over dt (src/subroutines/physics.f90:272-664): Ohmic
+ * resistivity (edge EMFs -eta*J*dt and constrained transport) if eta > 0,
+ * Navier-Stokes viscosity (momentum and energy fluxes) if nu > 0, then the
+ * ghost fill.  The reference calls it after boundary when nu > 0 or eta > 0
+ * (src/dumses.f90:103-105); dmx_advance does the same. */
+int  dmx_dissipation(dmx_ctx* ctx, double time, double dt);
+
 /* Runs nsteps of the reference time loop (src/dumses.f90:75-120) on the
  * device without host synchronisation per step: compute_dt, godunov,
- * [fargo], boundary, time += dt.  *time is read and updated; if dts is not
+ * [fargo], boundary, [dissipation], time += dt.  *time is read and updated; if dts is not
  * NULL it receives the nsteps timesteps. */
 int  dmx_advance(dmx_ctx* ctx, int32_t nsteps, double* time, double* dts);
 

@@ -1444,6 +1444,189 @@ void dmxo_fargo(dmxo* s, double dt)
             }
 }
 
+/* resistivity, physics.f90:296-459: edge EMFs -eta*J*dt from the face
+ * fields on [1, n+1]^3 (emfx/y/z zeroed first, :313), then
+ * constrained_transport (:457).  The ISO=0 energy fluxes of :363-456 go to
+ * flux(:,:,:,ip,:), which nothing reads before viscosity zeroes it or the
+ * next godunov overwrites it, so they are not formed here. */
+static void resistivity(dmxo* s, double dt)
+{
+    const double eta = s->p.eta, dx = s->dx, dy = s->dy, dz = s->dz;
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    memset(s->emfx, 0, sizeof(double) * s->n3);
+    memset(s->emfy, 0, sizeof(double) * s->n3);
+    memset(s->emfz, 0, sizeof(double) * s->n3);
+    for (int k = 1; k <= nz + 1; k++)
+        for (int j = 1; j <= ny + 1; j++)
+            for (int i = 1; i <= nx + 1; i++) {
+                double dBydx = (U(i, j, k, IB) - U(i - 1, j, k, IB)) / dx;
+                double dBzdx = (U(i, j, k, IC) - U(i - 1, j, k, IC)) / dx;
+                double dBxdy = (U(i, j, k, IA) - U(i, j - 1, k, IA)) / dy;
+                double dBzdy = (U(i, j, k, IC) - U(i, j - 1, k, IC)) / dy;
+                double dBxdz = (U(i, j, k, IA) - U(i, j, k - 1, IA)) / dz;
+                double dBydz = (U(i, j, k, IB) - U(i, j, k - 1, IB)) / dz;
+                double jx = dBzdy - dBydz;
+                double jy = dBxdz - dBzdx, jz = dBydx - dBxdy;
+                EMFX(i, j, k) = -(eta * jx * dt);
+                EMFY(i, j, k) = -(eta * jy * dt);
+                EMFZ(i, j, k) = -(eta * jz * dt);
+            }
+    constrained_transport(s);
+}
+
+/* momentum component v (IU..IW) of cell (i,j,k) over the density of (a,b,c) */
+#define VEL(i, j, k, v, a, b, c) (U(i, j, k, v) / U(a, b, c, IR))
+
+/* viscosity, physics.f90:463-663: fluxes on [1, n+1]^3 (flux zeroed first,
+ * :284), then the conservative update of momentum and energy, x then y then
+ * z.  The x-direction difference divides the momentum of cell i-1 by the
+ * density of cell i (physics.f90:499), as the reference does. */
+static void viscosity(dmxo* s, double dt)
+{
+    const double nu = s->p.nu, dx = s->dx, dy = s->dy, dz = s->dz;
+    const double two3rd = 0.66666666666666667;
+    const int nx = s->nx, ny = s->ny, nz = s->nz, iso = s->p.iso;
+    memset(s->flux, 0, sizeof(double) * 24 * s->n3);
+    for (int k = 1; k <= nz + 1; k++)
+        for (int j = 1; j <= ny + 1; j++)
+            for (int i = 1; i <= nx + 1; i++) {
+                double dudx[3] = {0, 0, 0}, dudy[3] = {0, 0, 0}, dudz[3] = {0, 0, 0};
+                double rho, u = 0, v = 0, w = 0, uR, uL, uRR, uRL, uLR, uLL;
+                double txx, tyy, tzz, txy, txz, tyz;
+                /* first direction */
+                rho = half * (U(i, j, k, IR) + U(i - 1, j, k, IR));
+                if (!iso) {
+                    u = half * (VEL(i, j, k, IU, i, j, k) + VEL(i - 1, j, k, IU, i - 1, j, k));
+                    v = half * (VEL(i, j, k, IV, i, j, k) + VEL(i - 1, j, k, IV, i - 1, j, k));
+                    w = half * (VEL(i, j, k, IW, i, j, k) + VEL(i - 1, j, k, IW, i - 1, j, k));
+                }
+                for (int d = 0; d < 3; d++) {
+                    uR = VEL(i, j, k, IU + d, i, j, k);
+                    uL = VEL(i - 1, j, k, IU + d, i, j, k);
+                    dudx[d] = (uR - uL) / dx;
+                }
+                for (int d = 0; d < 2; d++) {
+                    uRR = VEL(i, j + 1, k, IU + d, i, j + 1, k);
+                    uRL = VEL(i - 1, j + 1, k, IU + d, i - 1, j + 1, k);
+                    uLR = VEL(i, j - 1, k, IU + d, i, j - 1, k);
+                    uLL = VEL(i - 1, j - 1, k, IU + d, i - 1, j - 1, k);
+                    uR = uRR + uRL; uL = uLR + uLL;
+                    dudy[d] = forth * (uR - uL) / dy;
+                }
+                for (int d = 0; d < 3; d += 2) {
+                    uRR = VEL(i, j, k + 1, IU + d, i, j, k + 1);
+                    uRL = VEL(i - 1, j, k + 1, IU + d, i - 1, j, k + 1);
+                    uLR = VEL(i, j, k - 1, IU + d, i, j, k - 1);
+                    uLL = VEL(i - 1, j, k - 1, IU + d, i - 1, j, k - 1);
+                    uR = uRR + uRL; uL = uLR + uLL;
+                    dudz[d] = forth * (uR - uL) / dz;
+                }
+                txx = -(two3rd * nu * rho * (two * dudx[0] - dudy[1] - dudz[2]));
+                txy = -(nu * rho * (dudy[0] + dudx[1]));
+                txz = -(nu * rho * (dudz[0] + dudx[2]));
+                FLX(i, j, k, IU, 0) = txx * dt / dx;
+                FLX(i, j, k, IV, 0) = txy * dt / dx;
+                FLX(i, j, k, IW, 0) = txz * dt / dx;
+                if (!iso) {
+                    FLX(i, j, k, IP, 0) = u * txx + v * txy + w * txz;
+                    FLX(i, j, k, IP, 0) = FLX(i, j, k, IP, 0) * dt / dx;
+                }
+                /* second direction */
+                rho = half * (U(i, j, k, IR) + U(i, j - 1, k, IR));
+                if (!iso) {
+                    u = half * (VEL(i, j, k, IU, i, j, k) + VEL(i, j - 1, k, IU, i, j - 1, k));
+                    v = half * (VEL(i, j, k, IV, i, j, k) + VEL(i, j - 1, k, IV, i, j - 1, k));
+                    w = half * (VEL(i, j, k, IW, i, j, k) + VEL(i, j - 1, k, IW, i, j - 1, k));
+                }
+                for (int d = 0; d < 3; d++) {
+                    uR = VEL(i, j, k, IU + d, i, j, k);
+                    uL = VEL(i, j - 1, k, IU + d, i, j - 1, k);
+                    dudy[d] = (uR - uL) / dy;
+                }
+                for (int d = 0; d < 2; d++) {
+                    uRR = VEL(i + 1, j, k, IU + d, i + 1, j, k);
+                    uRL = VEL(i + 1, j - 1, k, IU + d, i + 1, j - 1, k);
+                    uLR = VEL(i - 1, j, k, IU + d, i - 1, j, k);
+                    uLL = VEL(i - 1, j - 1, k, IU + d, i - 1, j - 1, k);
+                    uR = uRR + uRL; uL = uLR + uLL;
+                    dudx[d] = forth * (uR - uL) / dx;
+                }
+                for (int d = 1; d < 3; d++) {
+                    uRR = VEL(i, j, k + 1, IU + d, i, j, k + 1);
+                    uRL = VEL(i, j - 1, k + 1, IU + d, i, j - 1, k + 1);
+                    uLR = VEL(i, j, k - 1, IU + d, i, j, k - 1);
+                    uLL = VEL(i, j - 1, k - 1, IU + d, i, j - 1, k - 1);
+                    uR = uRR + uRL; uL = uLR + uLL;
+                    dudz[d] = forth * (uR - uL) / dz;
+                }
+                tyy = -(two3rd * nu * rho * (two * dudy[1] - dudx[0] - dudz[2]));
+                txy = -(nu * rho * (dudy[0] + dudx[1]));
+                tyz = -(nu * rho * (dudz[1] + dudy[2]));
+                FLX(i, j, k, IU, 1) = txy * dt / dy;
+                FLX(i, j, k, IV, 1) = tyy * dt / dy;
+                FLX(i, j, k, IW, 1) = tyz * dt / dy;
+                if (!iso) {
+                    FLX(i, j, k, IP, 1) = u * txy + v * tyy + w * tyz;
+                    FLX(i, j, k, IP, 1) = FLX(i, j, k, IP, 1) * dt / dy;
+                }
+                /* third direction */
+                rho = half * (U(i, j, k, IR) + U(i, j, k - 1, IR));
+                if (!iso) {
+                    u = half * (VEL(i, j, k, IU, i, j, k) + VEL(i, j, k - 1, IU, i, j, k - 1));
+                    v = half * (VEL(i, j, k, IV, i, j, k) + VEL(i, j, k - 1, IV, i, j, k - 1));
+                    w = half * (VEL(i, j, k, IW, i, j, k) + VEL(i, j, k - 1, IW, i, j, k - 1));
+                }
+                for (int d = 0; d < 3; d++) {
+                    uR = VEL(i, j, k, IU + d, i, j, k);
+                    uL = VEL(i, j, k - 1, IU + d, i, j, k - 1);
+                    dudz[d] = (uR - uL) / dz;
+                }
+                for (int d = 0; d < 3; d += 2) {
+                    uRR = VEL(i + 1, j, k, IU + d, i + 1, j, k);
+                    uRL = VEL(i + 1, j, k - 1, IU + d, i + 1, j, k - 1);
+                    uLR = VEL(i - 1, j, k, IU + d, i - 1, j, k);
+                    uLL = VEL(i - 1, j, k - 1, IU + d, i - 1, j, k - 1);
+                    uR = uRR + uRL; uL = uLR + uLL;
+                    dudx[d] = forth * (uR - uL) / dx;
+                }
+                for (int d = 1; d < 3; d++) {
+                    uRR = VEL(i, j + 1, k, IU + d, i, j + 1, k);
+                    uRL = VEL(i, j + 1, k - 1, IU + d, i, j + 1, k - 1);
+                    uLR = VEL(i, j - 1, k, IU + d, i, j - 1, k);
+                    uLL = VEL(i, j - 1, k - 1, IU + d, i, j - 1, k - 1);
+                    uR = uRR + uRL; uL = uLR + uLL;
+                    dudy[d] = forth * (uR - uL) / dy;
+                }
+                tzz = -(two3rd * nu * rho * (two * dudz[2] - dudx[0] - dudy[1]));
+                txz = -(nu * rho * (dudz[0] + dudx[2]));
+                tyz = -(nu * rho * (dudz[1] + dudy[2]));
+                FLX(i, j, k, IU, 2) = txz * dt / dz;
+                FLX(i, j, k, IV, 2) = tyz * dt / dz;
+                FLX(i, j, k, IW, 2) = tzz * dt / dz;
+                if (!iso) {
+                    FLX(i, j, k, IP, 2) = u * txz + v * tyz + w * tzz;
+                    FLX(i, j, k, IP, 2) = FLX(i, j, k, IP, 2) * dt / dz;
+                }
+            }
+    for (int d = 0; d < 3; d++) {
+        int i0 = d == 0, j0 = d == 1, k0 = d == 2;
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++)
+                    for (int v = IU; v <= IP; v++)
+                        U(i, j, k, v) = U(i, j, k, v) + (FLX(i, j, k, v, d) - FLX(i + i0, j + j0, k + k0, v, d));
+    }
+}
+#undef VEL
+
+/* dissipation, physics.f90:272-290 */
+void dmxo_dissipation(dmxo* s, double time, double dt)
+{
+    if (s->p.eta > zero) resistivity(s, dt);
+    if (s->p.nu > zero) viscosity(s, dt);
+    dmxo_boundary(s, time, dt);
+}
+
 /* reference loop body, dumses.f90:80-120 */
 double dmxo_advance(dmxo* s, int nsteps, double time, double* dts)
 {
@@ -1452,6 +1635,7 @@ double dmxo_advance(dmxo* s, int nsteps, double time, double* dts)
         dmxo_godunov(s, time, dt);
         if (s->p.fargo) dmxo_fargo(s, dt);
         dmxo_boundary(s, time, dt);
+        if (s->p.nu > zero || s->p.eta > zero) dmxo_dissipation(s, time, dt);
         time = time + dt;
         if (dts) dts[n] = dt;
     }

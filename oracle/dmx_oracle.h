@@ -30,6 +30,7 @@ double dmxo_compute_dt(dmxo* s);                 /* physics.f90:18-132   */
 void   dmxo_godunov(dmxo* s, double time, double dt); /* scheme.f90:104  */
 void   dmxo_fargo(dmxo* s, double dt);           /* fargo.f90:17-124     */
 void   dmxo_boundary(dmxo* s, double time, double dt); /* boundary.f90:18 */
+void   dmxo_dissipation(dmxo* s, double time, double dt); /* physics.f90:272 */
 /* nsteps of the reference loop (dumses.f90:80-120); returns the new time */
 double dmxo_advance(dmxo* s, int nsteps, double time, double* dts);
 void   dmxo_history(dmxo* s, double out[9]);     /* history.f90:62-118   */

@@ -51,6 +51,7 @@ def lib():
         L.dmxo_godunov.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.dmxo_fargo.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.dmxo_boundary.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.dmxo_dissipation.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.dmxo_advance.restype = ctypes.c_double
         L.dmxo_advance.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp]
         L.dmxo_history.argtypes = [ctypes.c_void_p, dp]
@@ -99,6 +100,9 @@ class Oracle:
 
     def boundary(self, time, dt):
         lib().dmxo_boundary(self.h, time, dt)
+
+    def dissipation(self, time, dt):
+        lib().dmxo_dissipation(self.h, time, dt)
 
     def advance(self, nsteps: int, time: float = 0.0):
         dts = np.zeros(max(nsteps, 1))
@@ -167,7 +171,8 @@ def write_input(path: str, p: Params, restart: int = 1) -> None:
         f.write(f" nx={p.nx}\n ny={p.ny}\n nz={p.nz}\n")
         for k in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax"):
             f.write(f" {k}={_fmt(getattr(p, k))}\n")
-        f.write(f" Omega0={_fmt(p.omega0)}\n ciso={_fmt(p.ciso)}\n gamma={_fmt(p.gamma)}\n/\n")
+        f.write(f" Omega0={_fmt(p.omega0)}\n ciso={_fmt(p.ciso)}\n gamma={_fmt(p.gamma)}\n")
+        f.write(f" nu={_fmt(p.nu)}\n eta={_fmt(p.eta)}\n/\n")
         f.write("&output_params\n dtdump=1.d30\n dthist=-1.d0\n dtspec=-1.d0\n io_type='binary'\n/\n")
         f.write("&mpi_params\n nxslice=1\n nyslice=1\n nzslice=1\n/\n")
 

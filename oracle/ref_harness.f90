@@ -7,7 +7,8 @@
 ! injects the initial state through the reference's binary restart reader,
 ! src/subroutines/restart.f90:212-223) and then the same step order as the
 ! reference time loop (src/dumses.f90:75-105):
-!     compute_dt -> godunov -> [fargo_update] -> boundary -> time += dt
+!     compute_dt -> godunov -> [fargo_update] -> boundary -> [dissipation]
+!     -> time += dt
 ! but for a fixed number of steps instead of up to tlim, and it writes the
 ! state and the per-step dt as raw little-endian float64 streams.
 !
@@ -52,6 +53,7 @@ program ref_harness
      call godunov
      if (fargo) call fargo_update
      call boundary
+     if ((nu > 0.d0) .or. (eta > 0.d0)) call dissipation
      time = time + dt
      write(21) dt
      if (dumpevery > 0) then

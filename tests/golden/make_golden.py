@@ -58,6 +58,14 @@ def step_cases():
     yield "shear_iso", p, u, [1, 8]
     p, u = perturbed_shear(8, 16, 6, fargo=1, iso=1)
     yield "shear_fargo_iso", p, u, [1, 8]
+    # dissipation, physics.f90:272-664 (after boundary, dumses.f90:103-105)
+    yield "diss_both", base.replace(nu=2e-3, eta=3e-3), rs.uin, [1, 5]
+    yield "diss_eta_iso_zg", base.replace(eta=5e-3, iso=1, ciso=0.7,
+                                          bdtype=(P.ZERO_GRADIENT, P.PERIODIC, P.PERIODIC)), rs.uin, [1, 5]
+    yield "diss_nu_hll_minmod", base.replace(nu=4e-3, slope_type=1, riemann=P.HLL,
+                                             riemann2d=P.LLF), rs.uin, [1, 5]
+    p, u = perturbed_shear(8, 16, 6, fargo=1, iso=0)
+    yield "shear_fargo_diss", p.replace(nu=1e-4, eta=2e-4), u, [1, 8]
 
 
 def write_step(name, p, uin, steps):
@@ -175,15 +183,19 @@ def copy_mri_history():
     print(f"mri_history_reference: {vals.shape[0]} rows")
 
 
-def main():
-    copy_mri_history()
+def main(only=None):
+    """Writes every fixture, or only the step fixtures named in `only`."""
     if not (O.reference_available(0) and O.reference_available(1)):
         raise SystemExit("build the reference first: make -f oracle/ref.mk")
-    write_kernels()
+    if not only:
+        copy_mri_history()
+        write_kernels()
     for name, p, u, steps in step_cases():
-        write_step(name, p, u, steps)
-    write_loop()
+        if not only or name in only:
+            write_step(name, p, u, steps)
+    if not only:
+        write_loop()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -10,7 +10,8 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 STEP_FIXTURES = ["step_hlld_mc", "step_hll_llf_minmod", "step_llf_upwind_multid",
                  "step_upwind_hlla_flat", "step_hlld_zerogradient", "orszag_tang_16",
-                 "shear_nofargo", "shear_fargo", "shear_iso", "shear_fargo_iso"]
+                 "shear_nofargo", "shear_fargo", "shear_iso", "shear_fargo_iso",
+                 "diss_both", "diss_eta_iso_zg", "diss_nu_hll_minmod", "shear_fargo_diss"]
 
 
 def params_from_json(s) -> P.Params:

@@ -54,6 +54,7 @@ def test_struct_layout_matches_header(tmp_path):
     dict(gamma=1.0), dict(courant=0.0), dict(precision=2), dict(xmax=0.0),
     dict(bdtype=(P.SHEARINGBOX, P.ZERO_GRADIENT, P.PERIODIC), omega0=1.0),
     dict(nx=900, ny=900, nz=900),        # a float64 plane above 4 GiB
+    dict(nu=-1e-3), dict(eta=float("nan")),
 ])
 def test_invalid_parameters_rejected(bad):
     p = P.Params(nx=8, ny=8, nz=8).replace(**bad)

@@ -78,7 +78,8 @@ def test_dropin_reports_device_errors_on_the_host(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["orszag_tang_16", "step_hlld_zerogradient", "shear_fargo",
-                                  "shear_nofargo", "shear_fargo_iso"])
+                                  "shear_nofargo", "shear_fargo_iso", "diss_both",
+                                  "diss_eta_iso_zg", "shear_fargo_diss"])
 def test_dropin_driver_matches_reference_bitwise(name):
     p, u, dts, outs, full = load_step(name)
     if not os.path.exists(exe(p.iso)):

@@ -76,10 +76,13 @@ def test_fp64_matches_oracle_random_ragged(case):
     assert np.array_equal(got[m], want[m]), float(np.abs(got - want)[m].max())
 
 
-def test_abi_calls_equal_advance():
-    """compute_dt / godunov / fargo / boundary called one by one (the Fortran
-    drop-in path) give the same state as the device-resident loop."""
-    p, u, dts, outs, full = load_step("shear_fargo")
+@pytest.mark.parametrize("name", ["shear_fargo", "shear_fargo_diss"])
+def test_abi_calls_equal_advance(name):
+    """compute_dt / godunov / fargo / boundary / dissipation called one by
+    one (the Fortran drop-in path) give the same state as the
+    device-resident loop."""
+    p, u, dts, outs, full = load_step(name)
+    print(f"[gpu] ABI call sequence {name}", flush=True)
     a = abi.Domain(p, u)
     b = abi.Domain(p, u)
     t = 0.0
@@ -88,6 +91,8 @@ def test_abi_calls_equal_advance():
         a.godunov(t, dt)
         a.fargo(dt)
         a.boundary(t, dt)
+        if p.nu > 0 or p.eta > 0:
+            a.dissipation(t, dt)
         t = t + dt
     tb, _ = b.advance(3, 0.0)
     assert t == tb
@@ -109,13 +114,16 @@ def test_history_sums_match_oracle():
     assert abs(h_g[8]) <= 1e-13 * bscale and abs(h_o[8]) <= 1e-13 * bscale
 
 
-def test_fp32_bulk_close_to_fp64():
+@pytest.mark.parametrize("diss", [False, True])
+def test_fp32_bulk_close_to_fp64(diss):
     """float32 bulk with float64 face B / CT / pressure recovery: L1 error per
     variable after 10 steps of Orszag-Tang at 32^3 below 1e-5, relative to the
     L1 norm of the variable (rho, E) or of its vector (momentum, B: Bz starts
-    at zero, so its own norm is no scale); div B at float64 round-off."""
+    at zero, so its own norm is no scale); div B at float64 round-off.  With
+    diss, viscosity and resistivity are on."""
     pb = P.orszag_tang(32)
-    p64 = pb.params
+    p64 = pb.params.replace(nu=1e-3, eta=1e-3) if diss else pb.params
+    print(f"[gpu] fp32 vs fp64 diss={diss}", flush=True)
     p32 = p64.replace(precision=1)
     a = abi.Domain(p64, pb.uin)
     b = abi.Domain(p32, pb.uin)
@@ -139,3 +147,32 @@ def test_invalid_step_size_rejected():
     d = abi.Domain(p, u)
     with pytest.raises(abi.DmxError):
         d.godunov(0.0, -1.0)
+
+
+def test_fp32_shearing_box_fargo_first_orbit():
+    """The shearing-box MRI workload with FARGO at its full size (128 x 256 x
+    128, box 1 x 4 x 1, Omega = 1, beta = 1000): float32 bulk against
+    float64 over the first orbit (t = 2 pi, the linear phase), per-variable
+    L1 difference relative to the variable's (rho, E) or vector's (momentum,
+    B) L1 norm below 1e-5."""
+    pb = P.shearing_box(128, 256, 128, fargo=1)
+    print("[gpu] shearing box 128x256x128 fp32 vs fp64, one orbit", flush=True)
+    tend = 2.0 * np.pi
+    out = []
+    for prec in (0, 1):
+        d = abi.Domain(pb.params.replace(precision=prec), pb.uin)
+        t, n = 0.0, 0
+        while t < tend and n < 2000:
+            t, _ = d.advance(1, t)
+            n += 1
+        out.append((P.interior(d.download()), t, n))
+        d.close()
+    (ua, ta, na), (ub, tb, nb) = out
+    print(f"[gpu] steps fp64 {na} (t={ta:.4f}), fp32 {nb} (t={tb:.4f})")
+    mom = np.sqrt((ua[1:4] ** 2).sum(axis=0)).sum()
+    mag = np.sqrt((ua[5:8] ** 2).sum(axis=0)).sum()
+    norms = [np.abs(ua[0]).sum(), mom, mom, mom, np.abs(ua[4]).sum(), mag, mag, mag]
+    for v in range(8):
+        l1 = np.abs(ub[v] - ua[v]).sum() / norms[v]
+        print(f"[gpu] shearing box fp32 L1 var {v}: {l1:.3e}")
+        assert l1 < 1e-5, (v, l1)
