@@ -84,6 +84,7 @@ def run_gpu(pb, steps, warmup, precision):
     from dmx import abi
     p = pb.params.replace(precision=precision)
     d = abi.Domain(p, pb.uin)
+    tuned_ms = d.tune()                    # launch-shape autotuner (state unchanged)
     t, _ = d.advance(warmup, 0.0)          # ends with a stream synchronisation
     t0 = time.perf_counter()
     prof = d.profile(steps, t)             # ends with a stream synchronisation
@@ -91,7 +92,7 @@ def run_gpu(pb, steps, warmup, precision):
     dev_ms = d.last_advance_ms()
     state = d.download()
     d.close()
-    return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state)
+    return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state, tuned_ms=tuned_ms)
 
 
 def cpu_baseline(n=128, steps=3):

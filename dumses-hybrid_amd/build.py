@@ -26,6 +26,7 @@ UNITS = [
     ("dmx_f64.hip", ["-ffp-contract=off"]),
     ("dmx_f32.hip", ["-ffp-contract=fast", "-ffast-math", "-fno-finite-math-only"]),
     ("dmx_abi.cpp", []),
+    ("dmx_mlp.hip", ["-ffp-contract=fast"]),
 ]
 
 

@@ -42,7 +42,7 @@ static int validate(const dmx_params* p)
         set_error("unknown 2D Riemann solver id %d", p->riemann2d);
         return DMX_E_PARAM;
     }
-    if (p->slope_type < 0 || p->slope_type > 3) {
+    if (p->slope_type < 0 || p->slope_type > 4) {
         set_error("unknown slope type %d", p->slope_type);
         return DMX_E_PARAM;
     }
@@ -180,6 +180,23 @@ int dmx_last_advance_ms(dmx_ctx* c, double* ms)
     CTX_CHECK(c);
     if (!ms) { dmx::set_error("null ms"); return DMX_E_PARAM; }
     return c->e->last_advance_ms(ms);
+}
+int dmx_set_limiter(dmx_ctx* c, const float* theta, int32_t n)
+{
+    CTX_CHECK(c);
+    if (!theta) { dmx::set_error("null theta"); return DMX_E_PARAM; }
+    return c->e->set_limiter(theta, n);
+}
+int dmx_learned_slopes(dmx_ctx* c, float* out, double* fallback)
+{
+    CTX_CHECK(c);
+    if (!out) { dmx::set_error("null out"); return DMX_E_PARAM; }
+    return c->e->learned_slopes(out, fallback);
+}
+int dmx_tune(dmx_ctx* c, double* best_ms)
+{
+    CTX_CHECK(c);
+    return c->e->tune(best_ms);
 }
 int dmx_profile(dmx_ctx* c, int32_t nsteps, double* time, double* ms, int32_t nkernels)
 {

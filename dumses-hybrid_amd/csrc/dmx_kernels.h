@@ -62,6 +62,12 @@ template <typename T> struct Fields {
     T* s;           // NSLOT planes
     T* f;           // 18 planes
     T* em;          // 3 planes
+    // learned limiter (slope_type 4) only
+    float* lq;      // 8 planes: primitive variables of every cell
+    float* ls;      // 24 planes: limited slopes [direction][variable]
+    const unsigned short* lw;   // packed bf16 weights (see the limiter kernel)
+    const float* lb;            // biases b1 (64), b2 (64), b3 (8)
+    unsigned int* lcount;       // fallback counter
 };
 
 } // namespace dmx

@@ -52,7 +52,10 @@ def params_to_c(p: Params) -> DmxParams:
 EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compute_dt",
            "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_dissipation", "dmx_advance", "dmx_history",
            "dmx_history_shear", "dmx_write_dump", "dmx_read_dump",
-           "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_last_error",
+           "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_tune",
+           "dmx_set_limiter", "dmx_learned_slopes", "dmx_mlp_create", "dmx_mlp_destroy",
+           "dmx_mlp_set_params", "dmx_mlp_get_params", "dmx_mlp_forward", "dmx_mlp_loss_grad",
+           "dmx_mlp_load_data", "dmx_mlp_train", "dmx_mfma_selftest", "dmx_last_error",
            "dmx_version")
 
 _lib = None
@@ -91,6 +94,20 @@ def lib(path: str | None = None):
     L.dmx_read_dump.argtypes = [vp, ctypes.c_char_p, dp, dp]
     L.dmx_profile.argtypes = [vp, ctypes.c_int32, dp, dp, ctypes.c_int32]
     L.dmx_kernel_name.argtypes = [ctypes.c_int32]
+    L.dmx_tune.argtypes = [vp, dp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.dmx_set_limiter.argtypes = [vp, fp, ctypes.c_int32]
+    L.dmx_learned_slopes.argtypes = [vp, fp, dp]
+    L.dmx_mlp_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp)]
+    L.dmx_mlp_destroy.argtypes = [vp]
+    L.dmx_mlp_destroy.restype = None
+    L.dmx_mlp_set_params.argtypes = [vp, fp, ctypes.c_int32]
+    L.dmx_mlp_get_params.argtypes = [vp, fp, ctypes.c_int32]
+    L.dmx_mlp_forward.argtypes = [vp, fp, ctypes.c_int32, fp]
+    L.dmx_mlp_loss_grad.argtypes = [vp, fp, fp, ctypes.c_int32, fp, fp]
+    L.dmx_mlp_load_data.argtypes = [vp, fp, fp, ctypes.c_int64]
+    L.dmx_mlp_train.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, fp]
+    L.dmx_mfma_selftest.argtypes = [fp]
     L.dmx_kernel_name.restype = ctypes.c_char_p
     L.dmx_last_error.restype = ctypes.c_char_p
     L.dmx_version.restype = ctypes.c_char_p
@@ -101,6 +118,11 @@ def lib(path: str | None = None):
 def _ptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def _check(rc: int, what: str):
@@ -184,13 +206,34 @@ class Domain:
         _check(lib().dmx_last_advance_ms(self.h, ctypes.byref(ms)), "dmx_last_advance_ms")
         return ms.value
 
-    def profile(self, nsteps: int, time: float | None = None, ngroups: int = 9) -> dict:
+    def profile(self, nsteps: int, time: float | None = None, ngroups: int = 10) -> dict:
         """Mean device ms per step of each kernel group over nsteps."""
         t = ctypes.c_double(self.time if time is None else time)
         ms = np.zeros(ngroups)
         _check(lib().dmx_profile(self.h, nsteps, ctypes.byref(t), _ptr(ms), ngroups), "dmx_profile")
         self.time = t.value
         return {lib().dmx_kernel_name(k).decode(): float(ms[k]) for k in range(ngroups)}
+
+    def tune(self) -> float:
+        """Autotunes the face/edge launch shape on the current state
+        (dmx_tune); returns the best stage time in ms."""
+        ms = ctypes.c_double()
+        _check(lib().dmx_tune(self.h, ctypes.byref(ms)), "dmx_tune")
+        return ms.value
+
+    def set_limiter(self, theta) -> None:
+        """Weights of the learned limiter (slope_type 4), 7304 floats."""
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        _check(lib().dmx_set_limiter(self.h, _fptr(th), th.size), "dmx_set_limiter")
+
+    def learned_slopes(self):
+        """(slopes (3, 8, nz, ny, nx) float32, fallback fraction) of the
+        current state (dmx_learned_slopes)."""
+        p = self.p
+        out = np.zeros((3, 8, p.nz, p.ny, p.nx), dtype=np.float32)
+        fb = ctypes.c_double()
+        _check(lib().dmx_learned_slopes(self.h, _fptr(out), ctypes.byref(fb)), "dmx_learned_slopes")
+        return out, fb.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -202,3 +245,66 @@ class Domain:
             self.close()
         except Exception:
             pass
+
+
+class Mlp:
+    """The limiter network's trainer on the GPU (dmx_mlp_*)."""
+
+    def __init__(self, max_batch: int):
+        h = ctypes.c_void_p()
+        _check(lib().dmx_mlp_create(max_batch, ctypes.byref(h)), "dmx_mlp_create")
+        self.h = h
+        self.cap = max_batch
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        _check(lib().dmx_mlp_set_params(self.h, _fptr(th), th.size), "dmx_mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        from .learned import NPARAM
+        th = np.zeros(NPARAM, dtype=np.float32)
+        _check(lib().dmx_mlp_get_params(self.h, _fptr(th), th.size), "dmx_mlp_get_params")
+        return th
+
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 8), dtype=np.float32)
+        _check(lib().dmx_mlp_forward(self.h, _fptr(X), X.shape[0], _fptr(Y)), "dmx_mlp_forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        from .learned import NPARAM
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(NPARAM, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(lib().dmx_mlp_loss_grad(self.h, _fptr(X), _fptr(T), X.shape[0], ctypes.byref(loss), _fptr(g)),
+               "dmx_mlp_loss_grad")
+        return loss.value, g
+
+    def load_data(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        _check(lib().dmx_mlp_load_data(self.h, _fptr(X), _fptr(T), X.shape[0]), "dmx_mlp_load_data")
+
+    def train(self, steps: int, batch: int, lr: float, seed: int = 0) -> np.ndarray:
+        losses = np.zeros(max(steps, 1), dtype=np.float32)
+        _check(lib().dmx_mlp_train(self.h, steps, batch, lr, seed, _fptr(losses)), "dmx_mlp_train")
+        return losses[:steps]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dmx_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mfma_selftest() -> np.ndarray:
+    out = np.zeros((16, 16), dtype=np.float32)
+    _check(lib().dmx_mfma_selftest(_fptr(out)), "dmx_mfma_selftest")
+    return out

@@ -308,6 +308,51 @@ def random_smooth(nx: int, ny: int, nz: int, seed: int = 1, kmax: int = 2,
                    {"seed": seed})
 
 
+def _spectral_field(shape, g, kmax: int, slope: float = -2.0) -> np.ndarray:
+    """Periodic random-phase Gaussian field with power spectrum P(k) ~ k^slope
+    for 1 <= |k| <= kmax (k in units of the box), zero mean, unit rms."""
+    nz, ny, nx = shape
+    kz = np.fft.fftfreq(nz) * nz
+    ky = np.fft.fftfreq(ny) * ny
+    kx = np.fft.rfftfreq(nx) * nx
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    k = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+    amp = np.zeros_like(k)
+    m = (k >= 1.0) & (k <= kmax)
+    # P(k) per mode ~ k^slope / k^2 (shell area), amplitude = sqrt of that
+    amp[m] = np.sqrt(k[m] ** (slope - 2.0))
+    ph = g.uniform(0.0, 2.0 * np.pi, k.shape)
+    spec = amp * g.standard_normal(k.shape) * np.exp(1j * ph)
+    f = np.fft.irfftn(spec, s=shape, axes=(0, 1, 2))
+    f -= f.mean()
+    return f / max(np.sqrt((f * f).mean()), 1e-300)
+
+
+def turbulence(n, seed: int = 4242, mach: float = 0.5, beta: float = 10.0, kmax: int = 8,
+               **kw) -> Problem:
+    """Decaying MHD turbulence in the periodic unit box (n int or (nx,ny,nz)):
+    rho = 1, P = 1/gamma (sound speed 1), random-phase Gaussian velocity and
+    vector potential with power spectrum k^-2 for k <= kmax, velocity scaled
+    to rms Mach `mach`, field scaled to plasma beta `beta` (B_rms^2 =
+    2 P / beta); face B is the discrete curl of the potential."""
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    p = Params(nx=nx, ny=ny, nz=nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0,
+               zmin=0.0, zmax=1.0).replace(**kw)
+    g = np.random.Generator(np.random.PCG64(seed))
+    shape = (p.nz, p.ny, p.nx)
+    pres0 = 1.0 / p.gamma
+    v = [_spectral_field(shape, g, kmax) for _ in range(3)]
+    vrms = np.sqrt(sum((c * c).mean() for c in v))
+    v = [mach * c / vrms for c in v]
+    a = [_spectral_field(shape, g, kmax) for _ in range(3)]
+    bx, by, bz = curl_faces(p, *a)
+    brms = np.sqrt((bx * bx + by * by + bz * bz).mean())
+    scale = np.sqrt(2.0 * pres0 / beta) / max(brms, 1e-300)
+    one = np.ones(shape)
+    return Problem(p, assemble(p, one, v[0], v[1], v[2], pres0 * one, scale * bx, scale * by,
+                               scale * bz), "turbulence", {"seed": seed, "mach": mach, "beta": beta})
+
+
 def interior(u: np.ndarray, nvar: int = 8) -> np.ndarray:
     """The interior cells of the first nvar variables."""
     return u[:nvar, 3:-3, 3:-3, 3:-3]

@@ -74,7 +74,9 @@ extern "C" {
 typedef struct dmx_params {
     int32_t nx, ny, nz;          /* interior cells, each >= 3                 */
     int32_t iso;                 /* 1: isothermal (reference ISO=1 build)     */
-    int32_t slope_type;          /* 0 none, 1 minmod, 2 MC, 3 multidim        */
+    int32_t slope_type;          /* 0 none, 1 minmod, 2 MC, 3 multidim,
+                                    4 learned (dmx_set_limiter; no counterpart
+                                    in the reference)                         */
     int32_t riemann;             /* DMX_LLF | DMX_HLL | DMX_HLLD | DMX_UPWIND  */
     int32_t riemann2d;           /* DMX_LLF | DMX_HLLD | DMX_UPWIND | DMX_HLLA */
     int32_t bdtype[3];           /* DMX_BC_* per direction                    */
@@ -163,6 +165,50 @@ int  dmx_last_advance_ms(dmx_ctx* ctx, double* ms);
  * preprocessor (utils/pyutils/preproc/preprocess.py). */
 int  dmx_profile(dmx_ctx* ctx, int32_t nsteps, double* time, double* ms, int32_t nkernels);
 const char* dmx_kernel_name(int32_t id);
+
+/* Launch-shape autotuner: on the uploaded state, times the candidate shapes
+ * of the face/edge Riemann stage (one fused kernel marching 4, 8, 16 or 32
+ * z-planes per block, or separate face and edge kernels), keeps the fastest
+ * for this context and returns its time per launch in *best_ms (may be
+ * NULL).  The state is unchanged.  No counterpart in the reference's step:
+ * it takes the role of utils/pyutils/gpuconfig/gpuconfig.py, which sweeps
+ * OpenACC vector sizes by re-running the code. */
+int  dmx_tune(dmx_ctx* ctx, double* best_ms);
+
+/* ---- learned slope limiter (slope_type 4) ------------------------------
+ * No counterpart in the reference.  A 40-64-64-8 GELU network maps the
+ * scale-free features of the 5-cell stencil of the 8 primitive variables
+ * along one direction to the 8 slopes of the centre cell; it runs on bf16
+ * MFMA (float accumulation) before the trace, and a slope that leaves the TVD
+ * region, or a cell whose density or pressure would turn non-positive at a
+ * face, falls back to van Leer.  Parameters are 7304 floats: W1[40][64], b1,
+ * W2[64][64], b2, W3[64][8], b3 (row = input).  dumses-hybrid_amd/dmx/learned.py
+ * is the float64 reference of features, network and fallback. */
+int  dmx_set_limiter(dmx_ctx* ctx, const float* theta, int32_t n);
+/* Learned slopes of the current state: out receives 24*nx*ny*nz floats
+ * [direction][variable][k][j][i] (interior); *fallback (may be NULL) the
+ * fraction of cell-directions that fell back to van Leer. */
+int  dmx_learned_slopes(dmx_ctx* ctx, float* out, double* fallback);
+
+/* Training of the network on the GPU (hand-written float32 forward and
+ * backward kernels, mean-squared error, Adam). */
+typedef struct dmx_mlp dmx_mlp;
+int  dmx_mlp_create(int32_t max_batch, dmx_mlp** mlp);
+void dmx_mlp_destroy(dmx_mlp* mlp);
+int  dmx_mlp_set_params(dmx_mlp* mlp, const float* theta, int32_t n);   /* resets Adam */
+int  dmx_mlp_get_params(dmx_mlp* mlp, float* theta, int32_t n);
+/* Y[n][8] for X[n][40] (host arrays, n <= max_batch) */
+int  dmx_mlp_forward(dmx_mlp* mlp, const float* X, int32_t n, float* Y);
+/* loss = 0.5 * sum((Y - T)^2) / n and its gradient (7304 floats) */
+int  dmx_mlp_loss_grad(dmx_mlp* mlp, const float* X, const float* T, int32_t n, float* loss, float* grad);
+/* copies a data set of n rows (X[n][40], T[n][8]) to the device */
+int  dmx_mlp_load_data(dmx_mlp* mlp, const float* X, const float* T, int64_t n);
+/* `steps` Adam steps on minibatches of `batch` rows drawn from the loaded
+ * data; losses (may be NULL) receives the minibatch loss of every step */
+int  dmx_mlp_train(dmx_mlp* mlp, int32_t steps, int32_t batch, float lr, uint64_t seed, float* losses);
+/* D = A B (16x32 by 32x16, exact in bf16) through the bf16 MFMA lane layout
+ * the limiter uses; out[16*16] row-major, for a host-side check */
+int  dmx_mfma_selftest(float* out);
 
 const char* dmx_last_error(void);
 const char* dmx_version(void);

@@ -49,7 +49,7 @@ def test_struct_layout_matches_header(tmp_path):
 
 @needs_lib
 @pytest.mark.parametrize("bad", [
-    dict(nx=2), dict(riemann=7), dict(riemann2d=3), dict(slope_type=4),
+    dict(nx=2), dict(riemann=7), dict(riemann2d=3), dict(slope_type=5),
     dict(bdtype=(P.PERIODIC, P.SHEARINGBOX, P.PERIODIC)), dict(fargo=1, omega0=0.0),
     dict(gamma=1.0), dict(courant=0.0), dict(precision=2), dict(xmax=0.0),
     dict(bdtype=(P.SHEARINGBOX, P.ZERO_GRADIENT, P.PERIODIC), omega0=1.0),

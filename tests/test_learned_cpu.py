@@ -1,0 +1,88 @@
+"""The learned limiter's float64 reference (dmx/learned.py), pinned by
+mathematics rather than by restating it: finite differences for the
+gradient, exact linear profiles for the targets, conservation and div B for
+the restriction, invariances for the features."""
+import numpy as np
+import pytest
+
+from dmx import learned as L
+from dmx import problems as P
+
+
+def test_gelu_derivative_matches_finite_differences():
+    x = np.linspace(-6, 6, 241)
+    h = 1e-6
+    fd = (L.gelu(x + h) - L.gelu(x - h)) / (2 * h)
+    assert np.allclose(L.gelu_grad(x), fd, rtol=1e-7, atol=1e-9)
+    # tanh form of GELU: odd part and limits
+    assert abs(L.gelu(0.0)) == 0.0 and abs(L.gelu(8.0) - 8.0) < 1e-9 and abs(L.gelu(-8.0)) < 1e-9
+
+
+def test_gradient_matches_central_differences():
+    g = np.random.Generator(np.random.PCG64(3))
+    theta = L.init_params(1).astype(np.float64) + 0.01 * g.standard_normal(L.NPARAM)
+    X = g.standard_normal((17, L.NIN))
+    T = g.standard_normal((17, L.NOUT))
+    loss, grad = L.loss_and_grad(theta, X, T)
+    idx = g.choice(L.NPARAM, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(L.NPARAM)
+        e[i] = h
+        fd = (L.loss_and_grad(theta + e, X, T)[0] - L.loss_and_grad(theta - e, X, T)[0]) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, grad[i])
+
+
+def test_features_are_scale_free():
+    g = np.random.Generator(np.random.PCG64(5))
+    q5 = g.standard_normal((50, 5, 8)) + 3.0
+    X1, s1 = L.features(q5)
+    X2, s2 = L.features(7.5 * q5)
+    assert np.allclose(X1, X2, rtol=1e-13, atol=1e-15)
+    assert np.allclose(s2, 7.5 * s1)
+    flat = np.ones((4, 5, 8))
+    X3, s3 = L.features(flat)
+    assert (X3 == 0).all() and (s3 == 0).all()
+
+
+def test_zero_network_falls_back_to_van_leer():
+    g = np.random.Generator(np.random.PCG64(6))
+    q5 = np.abs(g.standard_normal((200, 5, 8))) + 0.5
+    out, fb = L.limited_slopes(np.zeros(L.NPARAM), q5)
+    vl = L.van_leer(q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2])
+    assert np.array_equal(out, vl)
+
+
+def test_restriction_conserves_and_keeps_div_b_zero():
+    pb = P.turbulence(16, seed=2, mach=1.0, beta=1.0)
+    uc = L.restrict_state(pb.uin)
+    fi, ci = P.interior(pb.uin), P.interior(uc)
+    for v in range(5):
+        assert abs(ci[v].sum() * 8 - fi[v].sum()) <= 1e-12 * np.abs(fi[v]).sum()
+    bx, by, bz = uc[5], uc[6], uc[7]
+    div = (bx[3:-3, 3:-3, 4:-2] - bx[3:-3, 3:-3, 3:-3]) + (by[3:-3, 4:-2, 3:-3] - by[3:-3, 3:-3, 3:-3]) \
+        + (bz[4:-2, 3:-3, 3:-3] - bz[3:-3, 3:-3, 3:-3])
+    assert np.abs(div).max() < 1e-13
+
+
+def test_targets_of_a_linear_profile_are_exact():
+    """rho = 1 + 0.1 x (linear, fine 16^3): the coarse x-slope of rho is 0.1
+    times the coarse width; the other directions have zero slope."""
+    pb = P.turbulence(16, seed=2, mach=0.0, beta=1e30)
+    p = pb.params
+    u = pb.uin.copy()
+    x = p.xmin + p.dx * (np.arange(p.nx + 6) - 2.5)
+    u[0] = 1.0 + 0.1 * x[None, None, :]
+    u[1:4] = 0.0
+    u[4] = 1.0 / (p.gamma - 1.0)
+    t = L.target_slopes(u, p.gamma)
+    assert np.allclose(t[0, 0], 0.1 * 2 * p.dx, rtol=1e-12)
+    assert np.abs(t[1, 0]).max() < 1e-14 and np.abs(t[2, 0]).max() < 1e-14
+
+
+def test_training_pairs_shapes():
+    pb = P.turbulence(16, seed=3)
+    X, T = L.training_pairs(pb.uin, pb.params.gamma)
+    assert X.shape == (3 * 8 ** 3, 40) and T.shape == (3 * 8 ** 3, 8)
+    assert np.isfinite(X).all() and np.isfinite(T).all()
+    assert np.abs(X).max() <= 1.0 + 1e-6
