@@ -1,0 +1,180 @@
+"""Training and evaluation of the learned slope limiter.
+
+    python -m dmx.train_limiter train [--out PATH] [--steps N]
+    python -m dmx.train_limiter eval  [--weights PATH] [--n 32]
+
+train: an ensemble of decaying-turbulence boxes of uneven sizes is run at
+fine resolution by the plain solver (float32 bulk, HLLD/HLLD, MC slopes),
+one context and HIP stream per box, all boxes advancing concurrently from
+host threads; snapshots of each fine state give coarse/fine training pairs
+(dmx.learned.training_pairs), and the network is trained on the GPU by the
+library's own forward/backward/Adam kernels (dmx_mlp_*).
+
+eval: from one fine initial state, runs (a) the plain solver at twice the
+resolution, restricted to the coarse grid (the reference), (b) the plain
+solver (MC) on the coarse grid and (c) the learned limiter on the coarse
+grid, to the same time, and reports the L1 difference of (b) and (c) from
+(a) per variable, with the wall time of each run.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+from . import abi
+from . import learned as L
+from . import problems as P
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+WEIGHTS = os.path.join(DATA, "limiter_weights.npz")
+
+
+def ensemble_boxes(nboxes: int = 12, seed: int = 4242):
+    """Fine sizes drawn from {32, 48, 64}^3 (coarse 16..32), random-phase
+    Gaussian v and A (k^-2, k <= 8 in coarse-grid units), Mach and beta
+    drawn from {0.5, 1} x {1, 10}."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    boxes = []
+    for b in range(nboxes):
+        n = int(g.choice([32, 48, 64]))
+        mach = float(g.choice([0.5, 1.0]))
+        beta = float(g.choice([1.0, 10.0]))
+        boxes.append(P.turbulence(n, seed=seed + b, mach=mach, beta=beta, kmax=max(4, n // 8),
+                                  precision=1))
+    return boxes
+
+
+def run_ensemble(boxes, nsnap: int = 5, every: int = 6):
+    """Advances every box concurrently (one context and stream each, one host
+    thread each) and returns the fine snapshots [(uin, gamma)]."""
+    snaps = [[] for _ in boxes]
+    errors = []
+
+    def work(b, pb):
+        try:
+            d = abi.Domain(pb.params, pb.uin)
+            t = 0.0
+            for _ in range(nsnap):
+                t, _ = d.advance(every, t)
+                snaps[b].append(d.download())
+            d.close()
+        except Exception as e:   # reported after the join
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(b, pb)) for b, pb in enumerate(boxes)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    if errors:
+        raise errors[0]
+    return [(u, boxes[b].params.gamma) for b in range(len(boxes)) for u in snaps[b]]
+
+
+def train(out: str = WEIGHTS, steps: int = 3000, batch: int = 4096, seed: int = 0, log=print):
+    t0 = time.perf_counter()
+    boxes = ensemble_boxes()
+    snaps = run_ensemble(boxes)
+    t1 = time.perf_counter()
+    Xs, Ts = zip(*(L.training_pairs(u, gam) for u, gam in snaps))
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    log(f"[train] {len(boxes)} boxes, {len(snaps)} snapshots, {X.shape[0]} samples "
+        f"(ensemble {t1 - t0:.1f} s, pairs {time.perf_counter() - t1:.1f} s)")
+    g = np.random.Generator(np.random.PCG64(seed))
+    perm = g.permutation(X.shape[0])
+    nval = min(20000, X.shape[0] // 10)
+    Xv, Tv = X[perm[:nval]], T[perm[:nval]]
+    Xt, Tt = X[perm[nval:]], T[perm[nval:]]
+    m = abi.Mlp(max(batch, nval))
+    m.set_params(L.init_params(seed))
+    m.load_data(Xt, Tt)
+    losses = []
+    for phase, (n, lr) in enumerate(((steps * 2 // 3, 2e-3), (steps - steps * 2 // 3, 5e-4))):
+        ls = m.train(n, batch, lr, seed=seed + 17 * phase)
+        losses.append(ls)
+        vl, _ = m.loss_grad(Xv, Tv)
+        log(f"[train] phase {phase}: {n} steps lr {lr:g}, last minibatch loss "
+            f"{ls[-50:].mean():.4e}, validation loss {vl:.4e}")
+    theta = m.get_params()
+    # the loss of predicting MC slopes, for scale
+    mc = np.zeros_like(Tv)
+    base = 0.5 * ((mc - Tv) ** 2).sum() / Tv.shape[0]
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    np.savez(out, theta=theta, losses=np.concatenate(losses), val_loss=np.float64(vl),
+             zero_loss=np.float64(base), samples=np.int64(X.shape[0]))
+    log(f"[train] wrote {out}: validation loss {vl:.4e} (zero-slope loss {base:.4e})")
+    m.close()
+    return theta
+
+
+def run_to(d, tend: float, max_steps: int = 100000):
+    """Advances a Domain to exactly tend (last step shortened) through the
+    step-by-step ABI; returns the number of steps."""
+    t, n = 0.0, 0
+    while t < tend and n < max_steps:
+        dt = min(d.compute_dt(), tend - t)
+        d.godunov(t, dt)
+        d.boundary(t, dt)
+        t = t + dt
+        n += 1
+    return n
+
+
+def evaluate(theta, n: int = 32, tend: float = 0.25, seed: int = 777, mach: float = 1.0,
+             beta: float = 1.0, log=print):
+    """L1 error (per variable, relative to the reference's L1 norm) of the
+    plain MC and the learned coarse runs against the restricted fine run."""
+    fine = P.turbulence(2 * n, seed=seed, mach=mach, beta=beta, kmax=max(4, n // 4), precision=1)
+    uc0 = L.restrict_state(fine.uin)
+    pc = P.Params(**{**fine.params.__dict__}).replace(nx=n, ny=n, nz=n)
+    res = {}
+    t0 = time.perf_counter()
+    df = abi.Domain(fine.params, fine.uin)
+    nf = run_to(df, tend)
+    ref = P.interior(L.restrict_state(df.download()))
+    res["fine_wall_s"] = time.perf_counter() - t0
+    res["fine_steps"] = nf
+    for name, st in (("mc", 2), ("learned", 4)):
+        t0 = time.perf_counter()
+        d = abi.Domain(pc.replace(slope_type=st), uc0)
+        if st == 4:
+            d.set_limiter(theta)
+        ns = run_to(d, tend)
+        u = P.interior(d.download())
+        res[f"{name}_wall_s"] = time.perf_counter() - t0
+        res[f"{name}_steps"] = ns
+        l1 = [float(np.abs(u[v] - ref[v]).sum() / max(np.abs(ref[v]).sum(), 1e-300)) for v in range(8)]
+        res[f"{name}_l1"] = l1
+        res[f"{name}_l1_mean"] = float(np.mean(l1))
+        if st == 4:
+            _, fb = d.learned_slopes()
+            res["learned_fallback"] = fb
+        d.close()
+    log(f"[eval] n={n} fine {2 * n}: mean L1 mc {res['mc_l1_mean']:.4e}, learned "
+        f"{res['learned_l1_mean']:.4e} (fallback {res['learned_fallback']:.3f})")
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=("train", "eval"))
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--n", type=int, default=32)
+    a = ap.parse_args(argv)
+    if a.cmd == "train":
+        train(a.out, a.steps)
+    else:
+        theta = np.load(a.weights)["theta"]
+        print(evaluate(theta, a.n))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
