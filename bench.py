@@ -85,6 +85,7 @@ def run_gpu(pb, steps, warmup, precision):
     p = pb.params.replace(precision=precision)
     d = abi.Domain(p, pb.uin)
     tuned_ms = d.tune()                    # launch-shape autotuner (state unchanged)
+    log(f"[bench] tuned face/edge stage: mode {d.shape[0]}, {d.shape[1]} planes, {tuned_ms:.3f} ms")
     t, _ = d.advance(warmup, 0.0)          # ends with a stream synchronisation
     t0 = time.perf_counter()
     prof = d.profile(steps, t)             # ends with a stream synchronisation
@@ -92,7 +93,7 @@ def run_gpu(pb, steps, warmup, precision):
     dev_ms = d.last_advance_ms()
     state = d.download()
     d.close()
-    return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state, tuned_ms=tuned_ms)
+    return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state, tuned_ms=tuned_ms, shape=d.shape)
 
 
 def cpu_baseline(n=128, steps=3):
@@ -208,6 +209,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "cells": cells, "parallelism": "single domain, 1 GPU"},
         "device_ms_per_step": r64["dev_ms"] / a.steps,
+        "tuned_flux_shape": {"f64": list(r64["shape"]), **({"f32": list(r32["shape"])} if "fp32" in out else {})},
         "roofline": roof,
         # the whole step against the compulsory state traffic: one read and one
         # write of the 8 stored words of every cell, 128 B per zone-cycle

@@ -193,10 +193,13 @@ int dmx_learned_slopes(dmx_ctx* c, float* out, double* fallback)
     if (!out) { dmx::set_error("null out"); return DMX_E_PARAM; }
     return c->e->learned_slopes(out, fallback);
 }
-int dmx_tune(dmx_ctx* c, double* best_ms)
+int dmx_tune(dmx_ctx* c, double* best_ms, int32_t* shape)
 {
     CTX_CHECK(c);
-    return c->e->tune(best_ms);
+    int sh[2] = {0, 0};
+    int rc = c->e->tune(best_ms, sh);
+    if (!rc && shape) { shape[0] = sh[0]; shape[1] = sh[1]; }
+    return rc;
 }
 int dmx_profile(dmx_ctx* c, int32_t nsteps, double* time, double* ms, int32_t nkernels)
 {

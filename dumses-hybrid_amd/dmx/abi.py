@@ -94,7 +94,7 @@ def lib(path: str | None = None):
     L.dmx_read_dump.argtypes = [vp, ctypes.c_char_p, dp, dp]
     L.dmx_profile.argtypes = [vp, ctypes.c_int32, dp, dp, ctypes.c_int32]
     L.dmx_kernel_name.argtypes = [ctypes.c_int32]
-    L.dmx_tune.argtypes = [vp, dp]
+    L.dmx_tune.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int32)]
     fp = ctypes.POINTER(ctypes.c_float)
     L.dmx_set_limiter.argtypes = [vp, fp, ctypes.c_int32]
     L.dmx_learned_slopes.argtypes = [vp, fp, dp]
@@ -216,9 +216,12 @@ class Domain:
 
     def tune(self) -> float:
         """Autotunes the face/edge launch shape on the current state
-        (dmx_tune); returns the best stage time in ms."""
+        (dmx_tune); returns the best stage time in ms and keeps the chosen
+        (mode, planes) in self.shape."""
         ms = ctypes.c_double()
-        _check(lib().dmx_tune(self.h, ctypes.byref(ms)), "dmx_tune")
+        sh = (ctypes.c_int32 * 2)()
+        _check(lib().dmx_tune(self.h, ctypes.byref(ms), sh), "dmx_tune")
+        self.shape = (int(sh[0]), int(sh[1]))
         return ms.value
 
     def set_limiter(self, theta) -> None:

@@ -167,13 +167,15 @@ int  dmx_profile(dmx_ctx* ctx, int32_t nsteps, double* time, double* ms, int32_t
 const char* dmx_kernel_name(int32_t id);
 
 /* Launch-shape autotuner: on the uploaded state, times the candidate shapes
- * of the face/edge Riemann stage (one fused kernel marching 4, 8, 16 or 32
- * z-planes per block, or separate face and edge kernels), keeps the fastest
- * for this context and returns its time per launch in *best_ms (may be
- * NULL).  The state is unchanged.  No counterpart in the reference's step:
+ * of the face/edge Riemann stage -- mode 0: one fused kernel marching 4, 8,
+ * 16 or 32 z-planes per block; mode 1: separate face and edge kernels;
+ * mode 2: the fused kernel carrying the previous plane's states in LDS
+ * (8, 16 or 32 planes) -- keeps the fastest for this context, and returns
+ * its time per launch in *best_ms and {mode, planes} in shape[2] (either may
+ * be NULL).  The state is unchanged.  No counterpart in the reference's step:
  * it takes the role of utils/pyutils/gpuconfig/gpuconfig.py, which sweeps
  * OpenACC vector sizes by re-running the code. */
-int  dmx_tune(dmx_ctx* ctx, double* best_ms);
+int  dmx_tune(dmx_ctx* ctx, double* best_ms, int32_t* shape);
 
 /* ---- learned slope limiter (slope_type 4) ------------------------------
  * No counterpart in the reference.  A 40-64-64-8 GELU network maps the
