@@ -44,6 +44,8 @@ def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
         "edges": 47 * wt + 3 * wt,
         "flux": 47 * wt + 18 * wt + 3 * wt,           # faces and edges in one pass
         "update_ct": 2 * (hydro + bfield) + 18 * wt + 3 * wt,
+        # learned limiter: state -> 8 float primitives -> 24 float slopes
+        "learned_limiter": hydro + bfield + 8 * 4 + 8 * 4 + 24 * 4,
         "boundary": 0.0,
     }
     return float(table.get(group, 0.0))
@@ -52,11 +54,13 @@ def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
 # bytes leaving L2 per interior cell and launch, (2 x FETCH_SIZE + WRITE_SIZE)
 # x 1 KiB from the two rocprofv3 counter runs at 256^3 summarised in
 # profiles/mhd_step_ot256_hbm_bytes.md (FETCH_SIZE scaled by 2.00, WRITE_SIZE
-# by 1.00 against the known-size calibration kernels of the same runs)
+# by 1.00 against the known-size calibration kernels of the same runs), for
+# the launch shapes the autotuner keeps at 256^3 (float64: fused flux kernel,
+# 8 planes; float32: carrying flux kernel, 8 planes)
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 756.2, "flux": 1647.4, "update_ct": 416.3, "compute_dt": 86.0,
+    "f64": {"trace": 756.2, "flux": 1651.1, "update_ct": 416.3, "compute_dt": 85.7,
             "boundary": 17.5},
-    "f32": {"trace": 479.4, "flux": 1031.0, "update_ct": 279.9, "compute_dt": 64.7,
+    "f32": {"trace": 476.0, "flux": 839.6, "update_ct": 279.9, "compute_dt": 64.6,
             "boundary": 15.1},
 }
 
@@ -77,13 +81,17 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
                 kernel_ms=ms, step_kernel_ms=prof)
 
 
-def run_gpu(pb, steps, warmup, precision):
+def run_gpu(pb, steps, warmup, precision, limiter=None):
     """Warm-up, then the timed region: `steps` steps of the instrumented
     advance (HIP events around every kernel group on the library's stream,
     read once after the run), bracketed by device synchronisation."""
     from dmx import abi
     p = pb.params.replace(precision=precision)
+    if limiter is not None:
+        p = p.replace(slope_type=4)
     d = abi.Domain(p, pb.uin)
+    if limiter is not None:
+        d.set_limiter(limiter)
     tuned_ms = d.tune()                    # launch-shape autotuner (state unchanged)
     log(f"[bench] tuned face/edge stage: mode {d.shape[0]}, {d.shape[1]} planes, {tuned_ms:.3f} ms")
     t, _ = d.advance(warmup, 0.0)          # ends with a stream synchronisation
@@ -149,6 +157,7 @@ def main():
                          "MRI with FARGO at 128x256x128 (a parity workload, reported for reference)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) to calibrate HBM byte counters")
     a = ap.parse_args()
@@ -199,6 +208,20 @@ def main():
                            l1_rel_vs_fp64=l1,
                            dtype="f32 bulk, f64 face-B/CT/pressure recovery/sums")
         log(f"[bench] float32: {v32:.4g} zone-cycles/s ({v32 / v64:.2f}x), L1 {l1}")
+    if not a.no_fp32 and not a.no_learned:
+        from dmx import train_limiter as TL
+        if os.path.exists(TL.WEIGHTS):
+            log("[bench] float32-bulk run with the learned limiter")
+            theta = np.load(TL.WEIGHTS)["theta"]
+            rl = run_gpu(pb, a.steps, a.warmup, 1, limiter=theta)
+            vl = cells * a.steps / rl["wall"]
+            ul = P.interior(rl["state"])
+            l1l = {names[v]: float(np.abs(ul[v] - u64[v]).sum() / norms[v]) for v in range(8)}
+            out["learned"] = dict(value=vl, ms_per_step=1e3 * rl["wall"] / a.steps, speedup_vs_fp64=vl / v64,
+                                  l1_rel_vs_fp64_mc=l1l, kernel_ms=rl["prof"],
+                                  note="float32 bulk with the learned limiter (slope_type 4); its L1 is "
+                                       "against the float64 MC run, so it measures the limiter change")
+            log(f"[bench] learned: {vl:.4g} zone-cycles/s ({vl / v64:.2f}x)")
     roof = roofline(r64["prof"], cells, "f64")
     if "fp32" in out:
         out["fp32"]["roofline"] = roofline(r32["prof"], cells, "f32")

@@ -299,6 +299,61 @@ int dmx_read_dump(dmx_ctx* c, const char* path, double* time, double* dt)
     return DMX_OK;
 }
 
+struct dmx_batch {
+    dmx::Batch* b;
+    int32_t n;
+};
+
+int dmx_batch_create(dmx_ctx* const* ctxs, int32_t n, dmx_batch** batch)
+{
+    if (!batch) { dmx::set_error("null batch"); return DMX_E_PARAM; }
+    *batch = nullptr;
+    if (!ctxs || n < 1) { dmx::set_error("batch: no contexts"); return DMX_E_PARAM; }
+    std::vector<dmx::Engine*> es(n);
+    for (int32_t b = 0; b < n; b++) {
+        if (!ctxs[b] || !ctxs[b]->e) { dmx::set_error("batch: null context %d", b); return DMX_E_STATE; }
+        for (int32_t a = 0; a < b; a++)
+            if (ctxs[a] == ctxs[b]) { dmx::set_error("batch: context %d repeats context %d", b, a); return DMX_E_PARAM; }
+        es[b] = ctxs[b]->e;
+    }
+    int rc = 0;
+    dmx::Batch* bt = ctxs[0]->p.precision == DMX_FP32 ? dmx::make_batch_f32(es.data(), n, &rc)
+                                                      : dmx::make_batch_f64(es.data(), n, &rc);
+    if (!bt) return rc ? rc : DMX_E_HIP;
+    *batch = new dmx_batch{bt, n};
+    return DMX_OK;
+}
+
+void dmx_batch_destroy(dmx_batch* batch)
+{
+    if (!batch) return;
+    delete batch->b;
+    delete batch;
+}
+
+int dmx_batch_advance(dmx_batch* batch, int32_t nsteps, double* times)
+{
+    if (!batch || !batch->b) { dmx::set_error("null batch"); return DMX_E_STATE; }
+    if (!times) { dmx::set_error("null times"); return DMX_E_PARAM; }
+    return batch->b->advance(nsteps, times);
+}
+
+int dmx_batch_tune(dmx_batch* batch, double* best_ms, int32_t* shape)
+{
+    if (!batch || !batch->b) { dmx::set_error("null batch"); return DMX_E_STATE; }
+    int sh[2] = {0, 0};
+    int rc = batch->b->tune(best_ms, sh);
+    if (!rc && shape) { shape[0] = sh[0]; shape[1] = sh[1]; }
+    return rc;
+}
+
+int dmx_batch_last_ms(dmx_batch* batch, double* ms)
+{
+    if (!batch || !batch->b) { dmx::set_error("null batch"); return DMX_E_STATE; }
+    if (!ms) { dmx::set_error("null ms"); return DMX_E_PARAM; }
+    return batch->b->last_ms(ms);
+}
+
 const char* dmx_kernel_name(int32_t id)
 {
     return (id >= 0 && id < dmx::KG_N) ? dmx::kKernelGroupNames[id] : "";

@@ -31,17 +31,29 @@ public:
     virtual int learned_slopes(float* out, double* fallback) = 0;
 };
 
+// Several contexts of one precision advanced together (dmx_batch_*): every
+// kernel of the step is one launch over the tiles of all boxes.
+class Batch {
+public:
+    virtual ~Batch() {}
+    virtual int advance(int nsteps, double* times) = 0;
+    virtual int tune(double* best_ms, int* shape) = 0;
+    virtual int last_ms(double* ms) = 0;
+};
+
 // kernel groups timed by dmx_profile
 enum KernelGroup { KG_DT = 0, KG_TRACE, KG_FACES, KG_EDGES, KG_SHEARFLUX, KG_UPDATE, KG_FARGO, KG_BOUNDARY,
-                   KG_FLUX, KG_DISSIPATION, KG_N };
+                   KG_FLUX, KG_DISSIPATION, KG_LEARNED, KG_N };
 static const char* const kKernelGroupNames[KG_N] = {
     "compute_dt", "trace", "faces", "edges", "shearing_flux", "update_ct", "fargo", "boundary", "flux",
-    "dissipation"};
+    "dissipation", "learned_limiter"};
 
 // set by the engines, read by dmx_last_error()
 void set_error(const char* fmt, ...);
 
 Engine* make_engine_f64(const dmx_params& p, int* rc);
 Engine* make_engine_f32(const dmx_params& p, int* rc);
+Batch* make_batch_f64(Engine* const* es, int n, int* rc);
+Batch* make_batch_f32(Engine* const* es, int n, int* rc);
 
 } // namespace dmx

@@ -16,4 +16,14 @@ Engine* make_engine_f32(const dmx_params& p, int* rc)
     }
     return e;
 }
+Batch* make_batch_f32(Engine* const* es, int n, int* rc)
+{
+    f32::BatchT* b = new f32::BatchT();
+    *rc = b->init(es, n);
+    if (*rc) {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
 } // namespace dmx

@@ -15,4 +15,14 @@ Engine* make_engine_f64(const dmx_params& p, int* rc)
     }
     return e;
 }
+Batch* make_batch_f64(Engine* const* es, int n, int* rc)
+{
+    f64::BatchT* b = new f64::BatchT();
+    *rc = b->init(es, n);
+    if (*rc) {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
 } // namespace dmx

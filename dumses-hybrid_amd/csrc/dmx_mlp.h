@@ -64,17 +64,22 @@ __device__ __forceinline__ float van_leer(float dl, float dr)
 // whose slope leaves the TVD region (sign of dl and dr, at most twice the
 // smaller of |dl|, |dr|, zero at an extremum) takes van Leer; if density or
 // pressure would be non-positive at a face, all 8 take van Leer.  Returns
-// true when the cell fell back.
-__device__ __forceinline__ bool limit_slopes(float y[8], const float q0[8], const float dl[8], const float dr[8])
+// true when the cell fell back; *nrep counts the variables that the TVD
+// rule replaced.
+__device__ __forceinline__ bool limit_slopes(float y[8], const float q0[8], const float dl[8], const float dr[8],
+                                             unsigned int* nrep)
 {
     float vl[8];
+    unsigned int r = 0;
 #pragma unroll
     for (int v = 0; v < 8; v++) {
         vl[v] = van_leer(dl[v], dr[v]);
         bool tvd = (dl[v] * dr[v] > 0.0f) && (y[v] * dl[v] > 0.0f)
                    && (fabsf(y[v]) <= 2.0f * fminf(fabsf(dl[v]), fabsf(dr[v])));
         y[v] = tvd ? y[v] : vl[v];
+        r += tvd ? 0u : 1u;
     }
+    *nrep = r;
     bool pos = (q0[0] - 0.5f * fabsf(y[0]) > 0.0f) && (q0[4] - 0.5f * fabsf(y[4]) > 0.0f);
     if (!pos) {
 #pragma unroll

@@ -55,8 +55,9 @@ EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compu
            "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_tune",
            "dmx_set_limiter", "dmx_learned_slopes", "dmx_mlp_create", "dmx_mlp_destroy",
            "dmx_mlp_set_params", "dmx_mlp_get_params", "dmx_mlp_forward", "dmx_mlp_loss_grad",
-           "dmx_mlp_load_data", "dmx_mlp_train", "dmx_mfma_selftest", "dmx_last_error",
-           "dmx_version")
+           "dmx_mlp_load_data", "dmx_mlp_train", "dmx_mfma_selftest", "dmx_batch_create",
+           "dmx_batch_destroy", "dmx_batch_advance", "dmx_batch_tune", "dmx_batch_last_ms",
+           "dmx_last_error", "dmx_version")
 
 _lib = None
 
@@ -108,6 +109,12 @@ def lib(path: str | None = None):
     L.dmx_mlp_load_data.argtypes = [vp, fp, fp, ctypes.c_int64]
     L.dmx_mlp_train.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, fp]
     L.dmx_mfma_selftest.argtypes = [fp]
+    L.dmx_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
+    L.dmx_batch_destroy.argtypes = [vp]
+    L.dmx_batch_destroy.restype = None
+    L.dmx_batch_advance.argtypes = [vp, ctypes.c_int32, dp]
+    L.dmx_batch_tune.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int32)]
+    L.dmx_batch_last_ms.argtypes = [vp, dp]
     L.dmx_kernel_name.restype = ctypes.c_char_p
     L.dmx_last_error.restype = ctypes.c_char_p
     L.dmx_version.restype = ctypes.c_char_p
@@ -206,7 +213,7 @@ class Domain:
         _check(lib().dmx_last_advance_ms(self.h, ctypes.byref(ms)), "dmx_last_advance_ms")
         return ms.value
 
-    def profile(self, nsteps: int, time: float | None = None, ngroups: int = 10) -> dict:
+    def profile(self, nsteps: int, time: float | None = None, ngroups: int = 11) -> dict:
         """Mean device ms per step of each kernel group over nsteps."""
         t = ctypes.c_double(self.time if time is None else time)
         ms = np.zeros(ngroups)
@@ -230,13 +237,14 @@ class Domain:
         _check(lib().dmx_set_limiter(self.h, _fptr(th), th.size), "dmx_set_limiter")
 
     def learned_slopes(self):
-        """(slopes (3, 8, nz, ny, nx) float32, fallback fraction) of the
-        current state (dmx_learned_slopes)."""
+        """(slopes (3, 8, nz, ny, nx) float32, fraction of cell-directions
+        with the positivity fallback, fraction of variable slopes replaced by
+        the TVD rule) of the current state (dmx_learned_slopes)."""
         p = self.p
         out = np.zeros((3, 8, p.nz, p.ny, p.nx), dtype=np.float32)
-        fb = ctypes.c_double()
-        _check(lib().dmx_learned_slopes(self.h, _fptr(out), ctypes.byref(fb)), "dmx_learned_slopes")
-        return out, fb.value
+        fb = np.zeros(2)
+        _check(lib().dmx_learned_slopes(self.h, _fptr(out), _ptr(fb)), "dmx_learned_slopes")
+        return out, float(fb[0]), float(fb[1])
 
     def close(self):
         if getattr(self, "h", None):
@@ -298,6 +306,53 @@ class Mlp:
     def close(self):
         if getattr(self, "h", None):
             lib().dmx_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Domains of one precision advanced together (dmx_batch_*): every kernel
+    of the step is one launch over the tiles of all boxes.  The domains keep
+    their state (download each one); they must stay open while the batch is
+    used."""
+
+    def __init__(self, domains):
+        self.domains = list(domains)
+        arr = (ctypes.c_void_p * len(self.domains))(*[d.h.value for d in self.domains])
+        h = ctypes.c_void_p()
+        _check(lib().dmx_batch_create(arr, len(self.domains), ctypes.byref(h)), "dmx_batch_create")
+        self.h = h
+        self.shape = None
+
+    def advance(self, nsteps: int, times=None) -> np.ndarray:
+        """nsteps steps of every box from times (default: each domain's
+        time); returns the end times and stores them in the domains."""
+        t = np.array([d.time for d in self.domains] if times is None else times, dtype=np.float64)
+        _check(lib().dmx_batch_advance(self.h, nsteps, _ptr(t)), "dmx_batch_advance")
+        for d, tb in zip(self.domains, t):
+            d.time = float(tb)
+        return t
+
+    def tune(self) -> float:
+        ms = ctypes.c_double()
+        sh = (ctypes.c_int32 * 2)()
+        _check(lib().dmx_batch_tune(self.h, ctypes.byref(ms), sh), "dmx_batch_tune")
+        self.shape = (int(sh[0]), int(sh[1]))
+        return ms.value
+
+    def last_ms(self) -> float:
+        ms = ctypes.c_double()
+        _check(lib().dmx_batch_last_ms(self.h, ctypes.byref(ms)), "dmx_batch_last_ms")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dmx_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

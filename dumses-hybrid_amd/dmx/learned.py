@@ -138,7 +138,8 @@ def limited_slopes(theta, q5):
     slope leaves the TVD region (sign of the one-sided differences, at most
     twice the smaller one, zero at an extremum) takes the van Leer slope, and
     a cell whose density or pressure would turn non-positive at a face takes
-    van Leer for all 8 variables.  Returns (slopes (n, 8), fell_back (n,))."""
+    van Leer for all 8 variables.  Returns (slopes (n, 8), fell_back (n,),
+    replaced (n, 8): the variables the TVD rule replaced)."""
     X, s = features(q5)
     y = forward(theta, X) * s
     dl = q5[:, 2] - q5[:, 1]
@@ -151,7 +152,7 @@ def limited_slopes(theta, q5):
     for v in (0, 4):
         pos &= (q0[:, v] - 0.5 * np.abs(out[:, v]) > 0)
     out = np.where(pos[:, None], out, vl)
-    return out, ~pos
+    return out, ~pos, ~tvd
 
 
 # ---------------------------------------------------------------------------

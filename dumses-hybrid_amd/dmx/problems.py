@@ -353,6 +353,16 @@ def turbulence(n, seed: int = 4242, mach: float = 0.5, beta: float = 10.0, kmax:
                                scale * bz), "turbulence", {"seed": seed, "mach": mach, "beta": beta})
 
 
+def ensemble(nboxes: int = 48, seed: int = 4242, sizes=(32, 48, 64, 96, 128), mach: float = 0.5,
+             beta: float = 10.0, kmax: int = 8, **kw) -> list:
+    """Independent periodic turbulence boxes of uneven sizes: box b has size
+    n_b^3 with n_b drawn from `sizes` by a PCG64 stream of `seed`, and the
+    initial fields of turbulence(n_b, seed + b, mach, beta, kmax)."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    ns = g.choice(np.asarray(sizes), size=nboxes)
+    return [turbulence(int(n), seed=seed + b, mach=mach, beta=beta, kmax=kmax, **kw) for b, n in enumerate(ns)]
+
+
 def interior(u: np.ndarray, nvar: int = 8) -> np.ndarray:
     """The interior cells of the first nvar variables."""
     return u[:nvar, 3:-3, 3:-3, 3:-3]

@@ -1,12 +1,16 @@
 """Summaries of rocprofv3 CSV output for the MHD step kernels.
 
     python -m dmx.rocprof_summary stats  <kernel_stats.csv>
-    python -m dmx.rocprof_summary pmc    <counter_collection.csv> [...]
+    python -m dmx.rocprof_summary trace  <kernel_trace.csv> [--last N]
+    python -m dmx.rocprof_summary pmc    <counter_collection.csv> [...] [--last N]
 
-`stats` prints the kernel-trace statistics table; `pmc` prints, per kernel,
-the mean of every collected counter per dispatch, plus the HBM byte
-calibration rows (torch fill / elementwise kernels of known size) when the
-run included them.
+`stats` prints the kernel-trace statistics table; `trace` prints the same
+from the per-dispatch trace; `pmc` prints, per kernel, the mean of every
+collected counter per dispatch, plus the HBM byte calibration rows (torch
+fill / elementwise kernels of known size) when the run included them.
+With --last N only the last N dispatches of each kernel count: the
+launch-shape autotuner runs every candidate before the timed steps, so the
+last dispatches are the steady-state ones of the chosen shape.
 """
 from __future__ import annotations
 
@@ -35,11 +39,34 @@ def stats(path: str) -> str:
     return "\n".join(out)
 
 
-def pmc(paths: list[str]) -> str:
+def trace(path: str, last: int | None = None) -> str:
+    runs = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        runs[short(r["Kernel_Name"])].append((int(r["Dispatch_Id"]),
+                                              (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+    rows = []
+    for k, v in runs.items():
+        v.sort()
+        d = [t for _, t in (v[-last:] if last else v)]
+        rows.append((sum(d), k, len(v), len(d), sum(d) / len(d), min(d), max(d)))
+    out = ["| kernel | calls | counted | mean us | min us | max us |", "|---|---:|---:|---:|---:|---:|"]
+    for _, k, n, c, mean, lo, hi in sorted(rows, reverse=True):
+        out.append(f"| `{k}` | {n} | {c} | {mean:.1f} | {lo:.1f} | {hi:.1f} |")
+    return "\n".join(out)
+
+
+def pmc(paths: list[str], last: int | None = None) -> str:
     vals = collections.defaultdict(lambda: collections.defaultdict(list))
     meta = {}
     for path in paths:
-        for r in csv.DictReader(open(path)):
+        rows = list(csv.DictReader(open(path)))
+        if last:
+            ids = collections.defaultdict(set)
+            for r in rows:
+                ids[r["Kernel_Name"]].add(int(r["Dispatch_Id"]))
+            keep = {k: set(sorted(v)[-last:]) for k, v in ids.items()}
+            rows = [r for r in rows if int(r["Dispatch_Id"]) in keep[r["Kernel_Name"]]]
+        for r in rows:
             k = short(r["Kernel_Name"])
             if k.startswith("__amd_rocclr"):
                 continue
@@ -60,10 +87,20 @@ def pmc(paths: list[str]) -> str:
 
 
 def main(argv):
-    if len(argv) < 3 or argv[1] not in ("stats", "pmc"):
+    if len(argv) < 3 or argv[1] not in ("stats", "trace", "pmc"):
         print(__doc__)
         return 2
-    print(stats(argv[2]) if argv[1] == "stats" else pmc(argv[2:]))
+    args, last = list(argv[2:]), None
+    if "--last" in args:
+        i = args.index("--last")
+        last = int(args[i + 1])
+        del args[i:i + 2]
+    if argv[1] == "stats":
+        print(stats(args[0]))
+    elif argv[1] == "trace":
+        print(trace(args[0], last))
+    else:
+        print(pmc(args, last))
     return 0
 
 

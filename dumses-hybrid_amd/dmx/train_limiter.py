@@ -152,11 +152,13 @@ def evaluate(theta, n: int = 32, tend: float = 0.25, seed: int = 777, mach: floa
         res[f"{name}_l1"] = l1
         res[f"{name}_l1_mean"] = float(np.mean(l1))
         if st == 4:
-            _, fb = d.learned_slopes()
+            _, fb, frep = d.learned_slopes()
             res["learned_fallback"] = fb
+            res["learned_tvd_replaced"] = frep
         d.close()
     log(f"[eval] n={n} fine {2 * n}: mean L1 mc {res['mc_l1_mean']:.4e}, learned "
-        f"{res['learned_l1_mean']:.4e} (fallback {res['learned_fallback']:.3f})")
+        f"{res['learned_l1_mean']:.4e} (positivity fallback {res['learned_fallback']:.3f}, "
+        f"TVD-replaced slopes {res['learned_tvd_replaced']:.3f})")
     return res
 
 

@@ -177,6 +177,30 @@ const char* dmx_kernel_name(int32_t id);
  * OpenACC vector sizes by re-running the code. */
 int  dmx_tune(dmx_ctx* ctx, double* best_ms, int32_t* shape);
 
+/* ---- batched ensemble -------------------------------------------------
+ * Several independent contexts of one precision advanced together: every
+ * kernel of the step (dt reduction, trace, face/edge stage, update, ghost
+ * fill) is ONE launch over the tiles of all boxes, which fills the GPU with
+ * many small boxes of uneven sizes where per-context launches would leave
+ * it idle.  Each box keeps its own dt and time, and its results are bit for
+ * bit those of dmx_advance on its own context.  No counterpart in the
+ * reference (one domain per run).  Requirements: every context uploaded,
+ * all of one precision and one Riemann solver pair, no shearing box, FARGO,
+ * dissipation or learned limiter.  The contexts must outlive the batch and
+ * are not used concurrently with it; their state stays in them (download
+ * each with dmx_download). */
+typedef struct dmx_batch dmx_batch;
+int  dmx_batch_create(dmx_ctx* const* ctxs, int32_t n, dmx_batch** batch);
+void dmx_batch_destroy(dmx_batch* batch);
+/* nsteps steps of every box; times[n] holds each box's start time on entry
+ * and its end time on return */
+int  dmx_batch_advance(dmx_batch* batch, int32_t nsteps, double* times);
+/* launch-shape autotuner of the batched face/edge stage (fused or carrying
+ * kernel, 4/8/16/32 planes per block); state unchanged, as dmx_tune */
+int  dmx_batch_tune(dmx_batch* batch, double* best_ms, int32_t* shape);
+/* milliseconds of the last dmx_batch_advance on the device */
+int  dmx_batch_last_ms(dmx_batch* batch, double* ms);
+
 /* ---- learned slope limiter (slope_type 4) ------------------------------
  * No counterpart in the reference.  A 40-64-64-8 GELU network maps the
  * scale-free features of the 5-cell stencil of the 8 primitive variables
@@ -188,8 +212,10 @@ int  dmx_tune(dmx_ctx* ctx, double* best_ms, int32_t* shape);
  * is the float64 reference of features, network and fallback. */
 int  dmx_set_limiter(dmx_ctx* ctx, const float* theta, int32_t n);
 /* Learned slopes of the current state: out receives 24*nx*ny*nz floats
- * [direction][variable][k][j][i] (interior); *fallback (may be NULL) the
- * fraction of cell-directions that fell back to van Leer. */
+ * [direction][variable][k][j][i] (interior); fallback (may be NULL)
+ * receives 2 values: the fraction of cell-directions that fell back to van
+ * Leer for all 8 variables (positivity) and the fraction of single-variable
+ * slopes that the TVD rule replaced by van Leer. */
 int  dmx_learned_slopes(dmx_ctx* ctx, float* out, double* fallback);
 
 /* Training of the network on the GPU (hand-written float32 forward and

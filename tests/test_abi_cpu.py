@@ -75,3 +75,19 @@ def test_null_context_is_an_error():
     assert lib.dmx_write_dump(None, b"/nonexistent/x.bin", 0.0, 0.0) == -5
     t = ctypes.c_double()
     assert lib.dmx_read_dump(None, b"/nonexistent/x.bin", ctypes.byref(t), ctypes.byref(dt)) == -5
+
+
+@needs_lib
+def test_batch_rejects_missing_contexts():
+    lib = abi.lib()
+    h = ctypes.c_void_p()
+    assert lib.dmx_batch_create(None, 2, ctypes.byref(h)) == -1
+    assert h.value is None
+    arr = (ctypes.c_void_p * 2)(None, None)
+    assert lib.dmx_batch_create(arr, 2, ctypes.byref(h)) == -5
+    assert b"null context" in lib.dmx_last_error()
+    t = (ctypes.c_double * 1)()
+    assert lib.dmx_batch_advance(None, 1, t) == -5
+    ms = ctypes.c_double()
+    assert lib.dmx_batch_last_ms(None, ctypes.byref(ms)) == -5
+    lib.dmx_batch_destroy(None)

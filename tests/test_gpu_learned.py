@@ -76,13 +76,15 @@ def test_limiter_kernel_matches_reference():
     print("[gpu] limiter kernel vs reference", flush=True)
     d = abi.Domain(p, pb.uin)
     d.set_limiter(theta)
-    got, fb = d.learned_slopes()
+    got, fb, frep = d.learned_slopes()
     u = d.download()
     agree = 0
     total = 0
+    rep_ref = 0
     for dd in range(3):
         q5 = L.stencils(u, p.gamma, dd).astype(np.float32).astype(np.float64)
-        ref, fell = L.limited_slopes(theta, q5)
+        ref, fell, replaced = L.limited_slopes(theta, q5)
+        rep_ref += replaced.sum()
         ref = ref.reshape(p.nz, p.ny, p.nx, 8).transpose(3, 0, 1, 2)
         gd = got[dd].astype(np.float64)
         X, s = L.features(q5)
@@ -90,9 +92,13 @@ def test_limiter_kernel_matches_reference():
         close = np.abs(gd - ref) <= 0.05 * scale
         agree += close.sum()
         total += close.size
-    print(f"[gpu] agreement {agree / total:.4f}, fallback fraction {fb:.4f}")
+    # the reference counts interior cells, the device [0, n+1]^3
+    frep_ref = rep_ref / total
+    print(f"[gpu] agreement {agree / total:.4f}, positivity fallback {fb:.4f}, TVD-replaced "
+          f"{frep:.4f} (reference {frep_ref:.4f})")
     assert agree / total > 0.98
     assert 0.0 <= fb <= 1.0
+    assert abs(frep - frep_ref) < 0.05
 
 
 def test_learned_step_conserves_and_keeps_div_b():
@@ -115,3 +121,19 @@ def test_learned_limiter_needs_weights():
     d = abi.Domain(pb.params.replace(slope_type=4), pb.uin)
     with pytest.raises(abi.DmxError, match="dmx_set_limiter"):
         d.advance(1, 0.0)
+
+
+def test_trained_limiter_gains_over_mc():
+    """The committed network (dmx/data/limiter_weights.npz, written by
+    python -m dmx.train_limiter train) against the plain MC limiter: from a
+    decaying-turbulence state (Mach 1, beta 1) not in the training ensemble,
+    32^3 runs to t = 0.25 compared with the 64^3 plain run restricted to
+    32^3.  The learned coarse run is closer to it than the MC coarse run."""
+    from dmx import train_limiter as TL
+    theta = np.load(TL.WEIGHTS)["theta"]
+    print("[gpu] learned limiter vs MC, 32^3 against 64^3", flush=True)
+    res = TL.evaluate(theta, n=32, tend=0.25, seed=777)
+    print(f"[gpu] mean L1: mc {res['mc_l1_mean']:.5f}, learned {res['learned_l1_mean']:.5f}; "
+          f"wall: fine {res['fine_wall_s']:.3f} s, mc {res['mc_wall_s']:.3f} s, "
+          f"learned {res['learned_wall_s']:.3f} s")
+    assert res["learned_l1_mean"] < res["mc_l1_mean"]

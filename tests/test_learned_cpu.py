@@ -48,7 +48,7 @@ def test_features_are_scale_free():
 def test_zero_network_falls_back_to_van_leer():
     g = np.random.Generator(np.random.PCG64(6))
     q5 = np.abs(g.standard_normal((200, 5, 8))) + 0.5
-    out, fb = L.limited_slopes(np.zeros(L.NPARAM), q5)
+    out, fb, rep = L.limited_slopes(np.zeros(L.NPARAM), q5)
     vl = L.van_leer(q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2])
     assert np.array_equal(out, vl)
 

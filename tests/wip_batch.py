@@ -1,0 +1,83 @@
+"""Batched ensemble (dmx_batch_*): boxes of uneven sizes, limiters and
+boundary types advanced by one launch per kernel give, box by box, the bits
+of dmx_advance on each box's own context, in both precisions and for every
+Riemann solver pair; what a batch cannot hold is rejected."""
+import numpy as np
+import pytest
+
+from dmx import abi
+from dmx import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def boxes(precision, riemann=P.HLLD, riemann2d=P.HLLD):
+    kw = dict(precision=precision, riemann=riemann, riemann2d=riemann2d)
+    return [
+        P.turbulence(16, seed=1, mach=1.0, beta=1.0, kmax=4, **kw),
+        P.random_smooth(24, 20, 12, seed=2, slope_type=1, **kw),
+        P.random_smooth(37, 9, 18, seed=3, slope_type=3,
+                        bdtype=(P.ZERO_GRADIENT, P.PERIODIC, P.ZERO_GRADIENT), **kw),
+        P.orszag_tang(8, **kw),
+    ]
+
+
+def singles(pbs, nsteps):
+    out = []
+    for pb in pbs:
+        d = abi.Domain(pb.params, pb.uin)
+        t, _ = d.advance(nsteps, 0.0)
+        out.append((t, d.download()))
+        d.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_equals_single_contexts(precision):
+    print(f"[gpu] batch of 4 uneven boxes, precision {precision}", flush=True)
+    pbs = boxes(precision)
+    ref = singles(pbs, 4)
+    doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    b = abi.Batch(doms)
+    ms = b.tune()                       # leaves the state unchanged
+    print(f"[gpu] tuned shape {b.shape}, {ms:.3f} ms")
+    b.advance(3, [0.0] * len(doms))
+    t = b.advance(1)                    # continues from the stored times
+    for d, tb, (ts, us) in zip(doms, t, ref):
+        assert tb == ts
+        assert np.array_equal(d.download(), us)
+    assert b.last_ms() > 0.0
+    b.close()
+    for d in doms:
+        d.close()
+
+
+@pytest.mark.parametrize("pair", [(P.LLF, P.LLF), (P.HLL, P.HLLA), (P.UPWIND, P.UPWIND), (P.HLLD, P.LLF)])
+def test_batch_solver_pairs(pair):
+    print(f"[gpu] batch solver pair {pair}", flush=True)
+    pbs = boxes(0, *pair)[:2]
+    ref = singles(pbs, 2)
+    doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    b = abi.Batch(doms)
+    b.advance(2, [0.0, 0.0])
+    for d, (ts, us) in zip(doms, ref):
+        assert d.time == ts
+        assert np.array_equal(d.download(), us)
+    b.close()
+
+
+def test_batch_rejections():
+    print("[gpu] batch rejections", flush=True)
+    a = abi.Domain(P.orszag_tang(8).params, P.orszag_tang(8).uin)
+    f32 = P.orszag_tang(8, precision=1)
+    c = abi.Domain(f32.params, f32.uin)
+    llf = P.orszag_tang(8, riemann=P.LLF)
+    e = abi.Domain(llf.params, llf.uin)
+    mri = P.mri_test(8, 16, 8)
+    s = abi.Domain(mri.params, mri.uin)
+    empty = abi.Domain(P.orszag_tang(8).params)
+    for group, what in (([a, c], b"precision"), ([a, e], b"Riemann"), ([a, s], b"shearing"),
+                        ([a, empty], b"dmx_upload"), ([a, a], b"repeats")):
+        with pytest.raises(abi.DmxError) as ei:
+            abi.Batch(group)
+        assert what.decode() in str(ei.value)
