@@ -1,12 +1,13 @@
 """Throughput of the MI355X MHD step on the 256^3 Orszag-Tang workload.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
 
 A step is one pass of the reference time loop (compute_dt, godunov with
 HLLD/HLLD fluxes and MC slopes, boundary) over the whole 256^3 periodic box,
 state resident in HBM.  value = zone-cycles/s of the float64 path (the
 reference's arithmetic); the float32-bulk path is reported beside it with its
-per-variable L1 difference from float64.  Prints one JSON line.
+per-variable L1 difference from float64, and the float32 path with the
+learned limiter beside that.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -149,8 +150,8 @@ def cpu_baseline(n=128, steps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--workload", choices=("ot", "shear"), default="ot",
                     help="ot: 3D Orszag-Tang n^3 (the metric's workload); shear: the shearing-box "
