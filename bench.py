@@ -105,6 +105,41 @@ def run_gpu(pb, steps, warmup, precision, limiter=None):
     return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state, tuned_ms=tuned_ms, shape=d.shape)
 
 
+def run_ensemble(steps=50, warmup=2, nboxes=48):
+    """The batched ensemble: 48 periodic turbulence boxes with sizes drawn
+    from {32,48,64,96,128}^3 (seed 4242, Mach 0.5, beta 10), float32 bulk,
+    advanced `steps` steps by one batch (dmx_batch_*, 2 streams, autotuned
+    shape) and, for comparison, by one context after another."""
+    from dmx import abi
+    from dmx import problems as P
+    boxes = P.ensemble(nboxes, precision=1)
+    cells = sum(pb.params.nx * pb.params.ny * pb.params.nz for pb in boxes)
+    doms = [abi.Domain(pb.params, pb.uin) for pb in boxes]
+    for d in doms:
+        d.advance(warmup, 0.0)
+    t0 = time.perf_counter()
+    for d in doms:
+        d.advance(steps)
+    seq = cells * steps / (time.perf_counter() - t0)
+    for d in doms:
+        d.close()
+    doms = [abi.Domain(pb.params, pb.uin) for pb in boxes]
+    b = abi.Batch(doms)
+    b.tune()
+    b.advance(warmup, [0.0] * nboxes)
+    t0 = time.perf_counter()
+    b.advance(steps)
+    bat = cells * steps / (time.perf_counter() - t0)
+    shape = b.shape
+    b.close()
+    for d in doms:
+        d.close()
+    return dict(workload=f"{nboxes} periodic turbulence boxes, sizes from {{32,48,64,96,128}}^3 "
+                         f"(seed 4242), float32 bulk, HLLD/HLLD, MC, {steps} steps",
+                cells=cells, batched=bat, per_context=seq, speedup=bat / seq, unit="zone-cycles/s",
+                shape=list(shape))
+
+
 def cpu_baseline(n=128, steps=3):
     """The compiled reference (its OpenMP -O3 build, run with 1 thread, its
     fastest setting) on a bounded sample of the same workload: Orszag-Tang
@@ -159,6 +194,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) to calibrate HBM byte counters")
     a = ap.parse_args()
@@ -241,6 +277,11 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": 128 * v64 / 1e9 / HBM_PEAK_GBS},
     }
     res.update(out)
+    if not a.no_ensemble:
+        log("[bench] batched ensemble")
+        res["ensemble"] = run_ensemble()
+        log(f"[bench] ensemble: batched {res['ensemble']['batched']:.4g}, per context "
+            f"{res['ensemble']['per_context']:.4g} zone-cycles/s")
     if not a.no_cpu:
         log("[bench] CPU baseline")
         res["cpu_baseline"] = cpu_baseline()

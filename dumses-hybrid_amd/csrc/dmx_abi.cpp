@@ -304,7 +304,7 @@ struct dmx_batch {
     int32_t n;
 };
 
-int dmx_batch_create(dmx_ctx* const* ctxs, int32_t n, dmx_batch** batch)
+int dmx_batch_create(dmx_ctx* const* ctxs, int32_t n, int32_t nstreams, dmx_batch** batch)
 {
     if (!batch) { dmx::set_error("null batch"); return DMX_E_PARAM; }
     *batch = nullptr;
@@ -317,8 +317,8 @@ int dmx_batch_create(dmx_ctx* const* ctxs, int32_t n, dmx_batch** batch)
         es[b] = ctxs[b]->e;
     }
     int rc = 0;
-    dmx::Batch* bt = ctxs[0]->p.precision == DMX_FP32 ? dmx::make_batch_f32(es.data(), n, &rc)
-                                                      : dmx::make_batch_f64(es.data(), n, &rc);
+    dmx::Batch* bt = ctxs[0]->p.precision == DMX_FP32 ? dmx::make_batch_f32(es.data(), n, nstreams, &rc)
+                                                      : dmx::make_batch_f64(es.data(), n, nstreams, &rc);
     if (!bt) return rc ? rc : DMX_E_HIP;
     *batch = new dmx_batch{bt, n};
     return DMX_OK;

@@ -53,7 +53,7 @@ void set_error(const char* fmt, ...);
 
 Engine* make_engine_f64(const dmx_params& p, int* rc);
 Engine* make_engine_f32(const dmx_params& p, int* rc);
-Batch* make_batch_f64(Engine* const* es, int n, int* rc);
-Batch* make_batch_f32(Engine* const* es, int n, int* rc);
+Batch* make_batch_f64(Engine* const* es, int n, int nstreams, int* rc);
+Batch* make_batch_f32(Engine* const* es, int n, int nstreams, int* rc);
 
 } // namespace dmx

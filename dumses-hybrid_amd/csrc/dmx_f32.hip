@@ -16,10 +16,10 @@ Engine* make_engine_f32(const dmx_params& p, int* rc)
     }
     return e;
 }
-Batch* make_batch_f32(Engine* const* es, int n, int* rc)
+Batch* make_batch_f32(Engine* const* es, int n, int nstreams, int* rc)
 {
     f32::BatchT* b = new f32::BatchT();
-    *rc = b->init(es, n);
+    *rc = b->init(es, n, nstreams);
     if (*rc) {
         delete b;
         return nullptr;

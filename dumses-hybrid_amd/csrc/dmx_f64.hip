@@ -15,10 +15,10 @@ Engine* make_engine_f64(const dmx_params& p, int* rc)
     }
     return e;
 }
-Batch* make_batch_f64(Engine* const* es, int n, int* rc)
+Batch* make_batch_f64(Engine* const* es, int n, int nstreams, int* rc)
 {
     f64::BatchT* b = new f64::BatchT();
-    *rc = b->init(es, n);
+    *rc = b->init(es, n, nstreams);
     if (*rc) {
         delete b;
         return nullptr;

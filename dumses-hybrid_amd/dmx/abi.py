@@ -109,7 +109,7 @@ def lib(path: str | None = None):
     L.dmx_mlp_load_data.argtypes = [vp, fp, fp, ctypes.c_int64]
     L.dmx_mlp_train.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, fp]
     L.dmx_mfma_selftest.argtypes = [fp]
-    L.dmx_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
+    L.dmx_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(vp)]
     L.dmx_batch_destroy.argtypes = [vp]
     L.dmx_batch_destroy.restype = None
     L.dmx_batch_advance.argtypes = [vp, ctypes.c_int32, dp]
@@ -321,11 +321,11 @@ class Batch:
     their state (download each one); they must stay open while the batch is
     used."""
 
-    def __init__(self, domains):
+    def __init__(self, domains, nstreams: int = 0):
         self.domains = list(domains)
         arr = (ctypes.c_void_p * len(self.domains))(*[d.h.value for d in self.domains])
         h = ctypes.c_void_p()
-        _check(lib().dmx_batch_create(arr, len(self.domains), ctypes.byref(h)), "dmx_batch_create")
+        _check(lib().dmx_batch_create(arr, len(self.domains), nstreams, ctypes.byref(h)), "dmx_batch_create")
         self.h = h
         self.shape = None
 

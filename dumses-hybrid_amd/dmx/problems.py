@@ -329,7 +329,7 @@ def _spectral_field(shape, g, kmax: int, slope: float = -2.0) -> np.ndarray:
 
 
 def turbulence(n, seed: int = 4242, mach: float = 0.5, beta: float = 10.0, kmax: int = 8,
-               **kw) -> Problem:
+               log=None, **kw) -> Problem:
     """Decaying MHD turbulence in the periodic unit box (n int or (nx,ny,nz)):
     rho = 1, P = 1/gamma (sound speed 1), random-phase Gaussian velocity and
     vector potential with power spectrum k^-2 for k <= kmax, velocity scaled
@@ -341,16 +341,28 @@ def turbulence(n, seed: int = 4242, mach: float = 0.5, beta: float = 10.0, kmax:
     g = np.random.Generator(np.random.PCG64(seed))
     shape = (p.nz, p.ny, p.nx)
     pres0 = 1.0 / p.gamma
-    v = [_spectral_field(shape, g, kmax) for _ in range(3)]
+
+    def field(name):
+        f = _spectral_field(shape, g, kmax)
+        if log:
+            log(f"[turbulence] {nx}x{ny}x{nz}: field {name} done")
+        return f
+
+    v = [field(f"v{c}") for c in "xyz"]
     vrms = np.sqrt(sum((c * c).mean() for c in v))
     v = [mach * c / vrms for c in v]
-    a = [_spectral_field(shape, g, kmax) for _ in range(3)]
+    a = [field(f"A{c}") for c in "xyz"]
     bx, by, bz = curl_faces(p, *a)
+    del a
+    if log:
+        log(f"[turbulence] {nx}x{ny}x{nz}: face field done")
     brms = np.sqrt((bx * bx + by * by + bz * bz).mean())
     scale = np.sqrt(2.0 * pres0 / beta) / max(brms, 1e-300)
     one = np.ones(shape)
-    return Problem(p, assemble(p, one, v[0], v[1], v[2], pres0 * one, scale * bx, scale * by,
-                               scale * bz), "turbulence", {"seed": seed, "mach": mach, "beta": beta})
+    u = assemble(p, one, v[0], v[1], v[2], pres0 * one, scale * bx, scale * by, scale * bz)
+    if log:
+        log(f"[turbulence] {nx}x{ny}x{nz}: state assembled")
+    return Problem(p, u, "turbulence", {"seed": seed, "mach": mach, "beta": beta})
 
 
 def ensemble(nboxes: int = 48, seed: int = 4242, sizes=(32, 48, 64, 96, 128), mach: float = 0.5,

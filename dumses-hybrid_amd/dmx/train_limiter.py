@@ -112,9 +112,10 @@ def train(out: str = WEIGHTS, steps: int = 3000, batch: int = 4096, seed: int = 
     return theta
 
 
-def run_to(d, tend: float, max_steps: int = 100000):
+def run_to(d, tend: float, max_steps: int = 100000, log=None, every: int = 100):
     """Advances a Domain to exactly tend (last step shortened) through the
-    step-by-step ABI; returns the number of steps."""
+    step-by-step ABI; returns the number of steps (log: progress every
+    `every` steps)."""
     t, n = 0.0, 0
     while t < tend and n < max_steps:
         dt = min(d.compute_dt(), tend - t)
@@ -122,29 +123,40 @@ def run_to(d, tend: float, max_steps: int = 100000):
         d.boundary(t, dt)
         t = t + dt
         n += 1
+        if log and n % every == 0:
+            log(f"[eval] step {n}, t = {t:.4f}")
     return n
 
 
 def evaluate(theta, n: int = 32, tend: float = 0.25, seed: int = 777, mach: float = 1.0,
-             beta: float = 1.0, log=print):
+             beta: float = 1.0, fine_precision: int = 1, kmax: int | None = None, log=print):
     """L1 error (per variable, relative to the reference's L1 norm) of the
-    plain MC and the learned coarse runs against the restricted fine run."""
-    fine = P.turbulence(2 * n, seed=seed, mach=mach, beta=beta, kmax=max(4, n // 4), precision=1)
+    plain MC and the learned coarse runs (float32 bulk) against the fine run
+    at twice the resolution (float32 bulk, or float64 with fine_precision
+    0), restricted to the coarse grid; with the wall time of each run."""
+    kmax = max(4, n // 4) if kmax is None else kmax
+    log(f"[eval] fine initial state {2 * n}^3")
+    fine = P.turbulence(2 * n, seed=seed, mach=mach, beta=beta, kmax=kmax, precision=fine_precision,
+                        log=log if n >= 128 else None)
     uc0 = L.restrict_state(fine.uin)
-    pc = P.Params(**{**fine.params.__dict__}).replace(nx=n, ny=n, nz=n)
+    pc = P.Params(**{**fine.params.__dict__}).replace(nx=n, ny=n, nz=n, precision=1)
     res = {}
     t0 = time.perf_counter()
+    log(f"[eval] fine run {2 * n}^3 (precision {fine_precision}) to t = {tend}")
     df = abi.Domain(fine.params, fine.uin)
-    nf = run_to(df, tend)
-    ref = P.interior(L.restrict_state(df.download()))
+    del fine
+    nf = run_to(df, tend, log=log if n >= 128 else None)
     res["fine_wall_s"] = time.perf_counter() - t0
     res["fine_steps"] = nf
+    ref = P.interior(L.restrict_state(df.download()))
+    df.close()
     for name, st in (("mc", 2), ("learned", 4)):
+        log(f"[eval] coarse {name} run {n}^3")
         t0 = time.perf_counter()
         d = abi.Domain(pc.replace(slope_type=st), uc0)
         if st == 4:
             d.set_limiter(theta)
-        ns = run_to(d, tend)
+        ns = run_to(d, tend, log=log if n >= 128 else None)
         u = P.interior(d.download())
         res[f"{name}_wall_s"] = time.perf_counter() - t0
         res[f"{name}_steps"] = ns
@@ -169,12 +181,21 @@ def main(argv=None):
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--tend", type=float, default=0.25)
+    ap.add_argument("--fine-precision", type=int, default=1, help="0: float64 fine run, 1: float32 bulk")
+    ap.add_argument("--kmax", type=int, default=None)
+    ap.add_argument("--json", default=None, help="also write the result to this file")
     a = ap.parse_args(argv)
     if a.cmd == "train":
         train(a.out, a.steps)
     else:
         theta = np.load(a.weights)["theta"]
-        print(evaluate(theta, a.n))
+        res = evaluate(theta, a.n, tend=a.tend, fine_precision=a.fine_precision, kmax=a.kmax)
+        print(res)
+        if a.json:
+            import json
+            with open(a.json, "w") as f:
+                json.dump(dict(n=a.n, tend=a.tend, fine_precision=a.fine_precision, **res), f)
     return 0
 
 

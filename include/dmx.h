@@ -188,9 +188,11 @@ int  dmx_tune(dmx_ctx* ctx, double* best_ms, int32_t* shape);
  * all of one precision and one Riemann solver pair, no shearing box, FARGO,
  * dissipation or learned limiter.  The contexts must outlive the batch and
  * are not used concurrently with it; their state stays in them (download
- * each with dmx_download). */
+ * each with dmx_download).  The boxes are dealt, by decreasing size, to
+ * nstreams groups (0: 2, at most 8), each launched on a stream of its own,
+ * so that kernels of different kinds of the groups overlap on the GPU. */
 typedef struct dmx_batch dmx_batch;
-int  dmx_batch_create(dmx_ctx* const* ctxs, int32_t n, dmx_batch** batch);
+int  dmx_batch_create(dmx_ctx* const* ctxs, int32_t n, int32_t nstreams, dmx_batch** batch);
 void dmx_batch_destroy(dmx_batch* batch);
 /* nsteps steps of every box; times[n] holds each box's start time on entry
  * and its end time on return */

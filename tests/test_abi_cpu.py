@@ -81,10 +81,10 @@ def test_null_context_is_an_error():
 def test_batch_rejects_missing_contexts():
     lib = abi.lib()
     h = ctypes.c_void_p()
-    assert lib.dmx_batch_create(None, 2, ctypes.byref(h)) == -1
+    assert lib.dmx_batch_create(None, 2, 0, ctypes.byref(h)) == -1
     assert h.value is None
     arr = (ctypes.c_void_p * 2)(None, None)
-    assert lib.dmx_batch_create(arr, 2, ctypes.byref(h)) == -5
+    assert lib.dmx_batch_create(arr, 2, 0, ctypes.byref(h)) == -5
     assert b"null context" in lib.dmx_last_error()
     t = (ctypes.c_double * 1)()
     assert lib.dmx_batch_advance(None, 1, t) == -5

@@ -52,6 +52,22 @@ def test_batch_equals_single_contexts(precision):
         d.close()
 
 
+@pytest.mark.parametrize("nstreams", [1, 3, 8])
+def test_batch_streams(nstreams):
+    """The boxes dealt to 1, 3 or 8 (capped at the box count) streams: the
+    same bits, and each box's time comes back in its own slot."""
+    print(f"[gpu] batch on {nstreams} streams", flush=True)
+    pbs = boxes(0)
+    ref = singles(pbs, 3)
+    doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    b = abi.Batch(doms, nstreams=nstreams)
+    t = b.advance(3, [0.0] * len(doms))
+    for d, tb, (ts, us) in zip(doms, t, ref):
+        assert tb == ts
+        assert np.array_equal(d.download(), us)
+    b.close()
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batch_tuned_shape(precision):
     """After the batch autotuner (state unchanged): float64 bit for bit; in
@@ -112,3 +128,6 @@ def test_batch_rejections():
         with pytest.raises(abi.DmxError) as ei:
             abi.Batch(group)
         assert what.decode() in str(ei.value)
+    with pytest.raises(abi.DmxError) as ei:
+        abi.Batch([a], nstreams=9)
+    assert "nstreams" in str(ei.value)
