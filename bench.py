@@ -66,6 +66,16 @@ MEASURED_TRAFFIC = {
 }
 
 
+# VALU-busy fraction of each kernel group at 256^3 (SQ_ACTIVE_INST_VALU x 4
+# / 1024 SIMDs / kernel cycles) from the rocprofv3 SQ counter run summarised
+# in profiles/mhd_step_ot256_sq_counters.md: the bound that applies where the
+# HBM fraction is low (bit-exact float64 division and square root)
+MEASURED_VALU_BUSY = {
+    "f64": {"trace": 0.364, "flux": 0.699, "update_ct": 0.122, "compute_dt": 0.803, "boundary": 0.121},
+    "f32": {"trace": 0.459, "flux": 0.581, "update_ct": 0.115, "compute_dt": 0.452, "boundary": 0.155},
+}
+
+
 def roofline(prof: dict, cells: int, prec: str) -> dict | None:
     """HBM roofline of the dominant kernel group of a profiled run."""
     if not prof:
@@ -78,6 +88,7 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
     traffic = meas * cells if meas is not None else None
     return dict(bound="hbm", kernel=dom, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=ach / HBM_PEAK_GBS, traffic=traffic, algo_bytes_per_cell=bpc,
+                valu_busy=MEASURED_VALU_BUSY.get(prec, {}).get(dom),
                 traffic_gbs=(traffic / (ms * 1e-3) / 1e9 if traffic and ms > 0 else None),
                 kernel_ms=ms, step_kernel_ms=prof)
 
@@ -94,7 +105,8 @@ def run_gpu(pb, steps, warmup, precision, limiter=None):
     if limiter is not None:
         d.set_limiter(limiter)
     tuned_ms = d.tune()                    # launch-shape autotuner (state unchanged)
-    log(f"[bench] tuned face/edge stage: mode {d.shape[0]}, {d.shape[1]} planes, {tuned_ms:.3f} ms")
+    log(f"[bench] tuned face/edge stage: mode {d.shape[0]}, {d.shape[1]} planes, {tuned_ms:.3f} ms; "
+        f"trace: mode {d.trace_shape[0]}, {d.trace_shape[1]} planes")
     t, _ = d.advance(warmup, 0.0)          # ends with a stream synchronisation
     t0 = time.perf_counter()
     prof = d.profile(steps, t)             # ends with a stream synchronisation
@@ -102,7 +114,8 @@ def run_gpu(pb, steps, warmup, precision, limiter=None):
     dev_ms = d.last_advance_ms()
     state = d.download()
     d.close()
-    return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state, tuned_ms=tuned_ms, shape=d.shape)
+    return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state, tuned_ms=tuned_ms, shape=d.shape,
+                trace_shape=d.trace_shape)
 
 
 def run_ensemble(steps=50, warmup=2, nboxes=48):
@@ -280,6 +293,8 @@ def main():
         "config": {"workload": workload, "cells": cells, "parallelism": "single domain, 1 GPU"},
         "device_ms_per_step": r64["dev_ms"] / a.steps,
         "tuned_flux_shape": {"f64": list(r64["shape"]), **({"f32": list(r32["shape"])} if "fp32" in out else {})},
+        "tuned_trace_shape": {"f64": list(r64["trace_shape"]),
+                              **({"f32": list(r32["trace_shape"])} if "fp32" in out else {})},
         "roofline": roof,
         # the whole step against the compulsory state traffic: one read and one
         # write of the 8 stored words of every cell, 128 B per zone-cycle

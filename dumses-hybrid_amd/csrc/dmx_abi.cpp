@@ -196,9 +196,10 @@ int dmx_learned_slopes(dmx_ctx* c, float* out, double* fallback)
 int dmx_tune(dmx_ctx* c, double* best_ms, int32_t* shape)
 {
     CTX_CHECK(c);
-    int sh[2] = {0, 0};
+    int sh[4] = {0, 0, 0, 0};
     int rc = c->e->tune(best_ms, sh);
-    if (!rc && shape) { shape[0] = sh[0]; shape[1] = sh[1]; }
+    if (!rc && shape)
+        for (int q = 0; q < 4; q++) shape[q] = sh[q];
     return rc;
 }
 int dmx_profile(dmx_ctx* c, int32_t nsteps, double* time, double* ms, int32_t nkernels)
@@ -341,9 +342,10 @@ int dmx_batch_advance(dmx_batch* batch, int32_t nsteps, double* times)
 int dmx_batch_tune(dmx_batch* batch, double* best_ms, int32_t* shape)
 {
     if (!batch || !batch->b) { dmx::set_error("null batch"); return DMX_E_STATE; }
-    int sh[2] = {0, 0};
+    int sh[4] = {0, 0, 0, 0};
     int rc = batch->b->tune(best_ms, sh);
-    if (!rc && shape) { shape[0] = sh[0]; shape[1] = sh[1]; }
+    if (!rc && shape)
+        for (int q = 0; q < 4; q++) shape[q] = sh[q];
     return rc;
 }
 

@@ -222,13 +222,15 @@ class Domain:
         return {lib().dmx_kernel_name(k).decode(): float(ms[k]) for k in range(ngroups)}
 
     def tune(self) -> float:
-        """Autotunes the face/edge launch shape on the current state
-        (dmx_tune); returns the best stage time in ms and keeps the chosen
-        (mode, planes) in self.shape."""
+        """Autotunes the trace and face/edge launch shapes on the current
+        state (dmx_tune); returns the best face/edge stage time in ms and
+        keeps the chosen (mode, planes) in self.shape (face/edge stage) and
+        self.trace_shape (trace)."""
         ms = ctypes.c_double()
-        sh = (ctypes.c_int32 * 2)()
+        sh = (ctypes.c_int32 * 4)()
         _check(lib().dmx_tune(self.h, ctypes.byref(ms), sh), "dmx_tune")
         self.shape = (int(sh[0]), int(sh[1]))
+        self.trace_shape = (int(sh[2]), int(sh[3]))
         return ms.value
 
     def set_limiter(self, theta) -> None:
@@ -340,7 +342,7 @@ class Batch:
 
     def tune(self) -> float:
         ms = ctypes.c_double()
-        sh = (ctypes.c_int32 * 2)()
+        sh = (ctypes.c_int32 * 4)()
         _check(lib().dmx_batch_tune(self.h, ctypes.byref(ms), sh), "dmx_batch_tune")
         self.shape = (int(sh[0]), int(sh[1]))
         return ms.value

@@ -167,14 +167,17 @@ int  dmx_profile(dmx_ctx* ctx, int32_t nsteps, double* time, double* ms, int32_t
 const char* dmx_kernel_name(int32_t id);
 
 /* Launch-shape autotuner: on the uploaded state, times the candidate shapes
- * of the face/edge Riemann stage -- mode 0: one fused kernel marching 4, 8,
- * 16 or 32 z-planes per block; mode 1: separate face and edge kernels;
- * mode 2: the fused kernel carrying the previous plane's states in LDS
- * (8, 16 or 32 planes) -- keeps the fastest for this context, and returns
- * its time per launch in *best_ms and {mode, planes} in shape[2] (either may
- * be NULL).  The state is unchanged.  No counterpart in the reference's step:
- * it takes the role of utils/pyutils/gpuconfig/gpuconfig.py, which sweeps
- * OpenACC vector sizes by re-running the code. */
+ * of the trace -- mode 0: a 16x4x4 cell tile per block; mode 1: a 16x16
+ * column tile marching 8, 16 or 32 z-planes -- and of the face/edge Riemann
+ * stage -- mode 0: one fused kernel marching 4, 8, 16 or 32 z-planes per
+ * block; mode 1: separate face and edge kernels; mode 2: the fused kernel
+ * carrying the previous plane's states in LDS (8, 16 or 32 planes) -- keeps
+ * the fastest of each for this context, and returns the face/edge stage's
+ * time per launch in *best_ms and {face/edge mode, planes, trace mode,
+ * planes} in shape[4] (either may be NULL).  Every shape gives the same
+ * results in float64.  The state is unchanged.  No counterpart in the
+ * reference's step: it takes the role of utils/pyutils/gpuconfig/gpuconfig.py,
+ * which sweeps OpenACC vector sizes by re-running the code. */
 int  dmx_tune(dmx_ctx* ctx, double* best_ms, int32_t* shape);
 
 /* ---- batched ensemble -------------------------------------------------
@@ -198,7 +201,8 @@ void dmx_batch_destroy(dmx_batch* batch);
  * and its end time on return */
 int  dmx_batch_advance(dmx_batch* batch, int32_t nsteps, double* times);
 /* launch-shape autotuner of the batched face/edge stage (fused or carrying
- * kernel, 4/8/16/32 planes per block); state unchanged, as dmx_tune */
+ * kernel, 4/8/16/32 planes per block); state unchanged, shape[4] as
+ * dmx_tune (the batch traces with the 16x4x4 tile: trace mode 0) */
 int  dmx_batch_tune(dmx_batch* batch, double* best_ms, int32_t* shape);
 /* milliseconds of the last dmx_batch_advance on the device */
 int  dmx_batch_last_ms(dmx_batch* batch, double* ms);

@@ -194,3 +194,37 @@ def test_fp32_shearing_box_fargo_first_orbit():
         l1 = np.abs(ub[v] - ua[v]).sum() / norms[v]
         print(f"[gpu] shearing box fp32 L1 var {v}: {l1:.3e}")
         assert l1 < 1e-5, (v, l1)
+
+
+@pytest.mark.parametrize("kt", [3, 16])
+def test_trace_z_march_bit_identical(kt, monkeypatch):
+    """The z-marching trace (DMX_TRACE_MODE=1, kt planes per block: 3
+    wraps the LDS rings mid-block on a ragged domain) gives the bits of the
+    3D-tile trace in float64, with the shearing-box and
+    multidimensional-limiter branches; in float32 (contraction under
+    -ffast-math may round the inlined copies differently) to 1e-5 of each
+    variable's largest magnitude after 3 steps."""
+    cases = [P.random_smooth(21, 13, 10, seed=31, slope_type=3),
+             P.random_smooth(17, 9, 23, seed=32, slope_type=2,
+                             bdtype=(P.ZERO_GRADIENT, P.PERIODIC, P.ZERO_GRADIENT)),
+             P.mri_test(16, 32, 16)]
+    for prec in (0, 1):
+        for pb in cases:
+            p = pb.params.replace(precision=prec)
+            print(f"[gpu] z-march trace kt={kt} precision {prec} {p.nx}x{p.ny}x{p.nz}", flush=True)
+            out = []
+            for mode in ("0", "1"):
+                monkeypatch.setenv("DMX_TRACE_MODE", mode)
+                monkeypatch.setenv("DMX_TRACE_KT", str(kt))
+                d = abi.Domain(p, pb.uin)
+                d.advance(3, 0.0)
+                out.append(d.download())
+                d.close()
+            same = np.array_equal(out[0], out[1])
+            print(f"[gpu]   bit-identical: {same}")
+            if prec == 0:
+                assert same
+            else:
+                a, b = P.interior(out[0]), P.interior(out[1])
+                for v in range(8):
+                    assert np.abs(a[v] - b[v]).max() <= 1e-5 * (np.abs(a[v]).max() + 1e-30), v
