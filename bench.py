@@ -57,11 +57,12 @@ def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
 # profiles/mhd_step_ot256_hbm_bytes.md (FETCH_SIZE scaled by 2.00, WRITE_SIZE
 # by 1.00 against the known-size calibration kernels of the same runs), for
 # the launch shapes the autotuner keeps at 256^3 (float64: fused flux kernel,
-# 8 planes; float32: carrying flux kernel, 8 planes)
+# 8 planes; float32: carrying flux kernel, 8 planes; z-marching trace, 16
+# planes)
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 756.2, "flux": 1651.1, "update_ct": 416.3, "compute_dt": 85.7,
+    "f64": {"trace": 638.9, "flux": 1651.6, "update_ct": 368.9, "compute_dt": 85.9,
             "boundary": 17.5},
-    "f32": {"trace": 476.0, "flux": 839.6, "update_ct": 279.9, "compute_dt": 64.6,
+    "f32": {"trace": 393.6, "flux": 837.5, "update_ct": 248.2, "compute_dt": 64.8,
             "boundary": 15.1},
 }
 
@@ -71,8 +72,8 @@ MEASURED_TRAFFIC = {
 # in profiles/mhd_step_ot256_sq_counters.md: the bound that applies where the
 # HBM fraction is low (bit-exact float64 division and square root)
 MEASURED_VALU_BUSY = {
-    "f64": {"trace": 0.364, "flux": 0.699, "update_ct": 0.122, "compute_dt": 0.803, "boundary": 0.121},
-    "f32": {"trace": 0.459, "flux": 0.581, "update_ct": 0.115, "compute_dt": 0.452, "boundary": 0.155},
+    "f64": {"trace": 0.407, "flux": 0.701, "update_ct": 0.130, "compute_dt": 0.783, "boundary": 0.118},
+    "f32": {"trace": 0.325, "flux": 0.564, "update_ct": 0.128, "compute_dt": 0.442, "boundary": 0.158},
 }
 
 
@@ -93,7 +94,14 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
                 kernel_ms=ms, step_kernel_ms=prof)
 
 
-def run_gpu(pb, steps, warmup, precision, limiter=None):
+# the launch shapes the autotuner keeps at 256^3 on MI355X ((face/edge mode,
+# planes), (trace mode, planes)); --no-tune applies them without timing, so
+# that profiler runs measure the kernels of the timed run (under a profiler
+# the autotuner's timings, and so its choice, can change)
+TUNED_256 = {0: ((0, 8), (1, 16)), 1: ((2, 8), (1, 16))}
+
+
+def run_gpu(pb, steps, warmup, precision, limiter=None, tune=True):
     """Warm-up, then the timed region: `steps` steps of the instrumented
     advance (HIP events around every kernel group on the library's stream,
     read once after the run), bracketed by device synchronisation."""
@@ -104,9 +112,14 @@ def run_gpu(pb, steps, warmup, precision, limiter=None):
     d = abi.Domain(p, pb.uin)
     if limiter is not None:
         d.set_limiter(limiter)
-    tuned_ms = d.tune()                    # launch-shape autotuner (state unchanged)
-    log(f"[bench] tuned face/edge stage: mode {d.shape[0]}, {d.shape[1]} planes, {tuned_ms:.3f} ms; "
-        f"trace: mode {d.trace_shape[0]}, {d.trace_shape[1]} planes")
+    if tune:
+        tuned_ms = d.tune()                # launch-shape autotuner (state unchanged)
+        log(f"[bench] tuned face/edge stage: mode {d.shape[0]}, {d.shape[1]} planes, {tuned_ms:.3f} ms; "
+            f"trace: mode {d.trace_shape[0]}, {d.trace_shape[1]} planes")
+    else:
+        d.set_shape(*TUNED_256[precision])
+        tuned_ms = None
+        log(f"[bench] fixed shapes: face/edge {d.shape}, trace {d.trace_shape}")
     t, _ = d.advance(warmup, 0.0)          # ends with a stream synchronisation
     t0 = time.perf_counter()
     prof = d.profile(steps, t)             # ends with a stream synchronisation
@@ -218,6 +231,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="apply the launch shapes the autotuner keeps at 256^3 instead of tuning")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) to calibrate HBM byte counters")
     a = ap.parse_args()
@@ -248,13 +263,13 @@ def main():
         workload = f"3D Orszag-Tang {n}^3 periodic, HLLD/HLLD, MC slopes, CFL 0.7"
     cells = pb.params.nx * pb.params.ny * pb.params.nz
     log("[bench] float64 run")
-    r64 = run_gpu(pb, a.steps, a.warmup, 0)
+    r64 = run_gpu(pb, a.steps, a.warmup, 0, tune=not a.no_tune)
     v64 = cells * a.steps / r64["wall"]
     log(f"[bench] float64: {v64:.4g} zone-cycles/s, {1e3 * r64['wall'] / a.steps:.3f} ms/step")
     out = {}
     if not a.no_fp32:
         log("[bench] float32-bulk run")
-        r32 = run_gpu(pb, a.steps, a.warmup, 1)
+        r32 = run_gpu(pb, a.steps, a.warmup, 1, tune=not a.no_tune)
         v32 = cells * a.steps / r32["wall"]
         u64, u32 = P.interior(r64["state"]), P.interior(r32["state"])
         names = ["rho", "rhou", "rhov", "rhow", "E", "Bx", "By", "Bz"]
@@ -273,7 +288,7 @@ def main():
         if os.path.exists(TL.WEIGHTS):
             log("[bench] float32-bulk run with the learned limiter")
             theta = np.load(TL.WEIGHTS)["theta"]
-            rl = run_gpu(pb, a.steps, a.warmup, 1, limiter=theta)
+            rl = run_gpu(pb, a.steps, a.warmup, 1, limiter=theta, tune=not a.no_tune)
             vl = cells * a.steps / rl["wall"]
             ul = P.interior(rl["state"])
             l1l = {names[v]: float(np.abs(ul[v] - u64[v]).sum() / norms[v]) for v in range(8)}

@@ -202,6 +202,13 @@ int dmx_tune(dmx_ctx* c, double* best_ms, int32_t* shape)
         for (int q = 0; q < 4; q++) shape[q] = sh[q];
     return rc;
 }
+int dmx_set_shape(dmx_ctx* c, const int32_t* shape)
+{
+    CTX_CHECK(c);
+    if (!shape) { dmx::set_error("null shape"); return DMX_E_PARAM; }
+    int sh[4] = {shape[0], shape[1], shape[2], shape[3]};
+    return c->e->set_shape(sh);
+}
 int dmx_profile(dmx_ctx* c, int32_t nsteps, double* time, double* ms, int32_t nkernels)
 {
     CTX_CHECK(c);

@@ -27,6 +27,7 @@ public:
     virtual int last_advance_ms(double* ms) = 0;
     virtual int profile(int nsteps, double* time, double* ms, int nk) = 0;
     virtual int tune(double* best_ms, int* shape) = 0;
+    virtual int set_shape(const int* shape) = 0;
     virtual int set_limiter(const float* theta, int n) = 0;
     virtual int learned_slopes(float* out, double* fallback) = 0;
 };

@@ -52,7 +52,7 @@ def params_to_c(p: Params) -> DmxParams:
 EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compute_dt",
            "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_dissipation", "dmx_advance", "dmx_history",
            "dmx_history_shear", "dmx_write_dump", "dmx_read_dump",
-           "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_tune",
+           "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_tune", "dmx_set_shape",
            "dmx_set_limiter", "dmx_learned_slopes", "dmx_mlp_create", "dmx_mlp_destroy",
            "dmx_mlp_set_params", "dmx_mlp_get_params", "dmx_mlp_forward", "dmx_mlp_loss_grad",
            "dmx_mlp_load_data", "dmx_mlp_train", "dmx_mfma_selftest", "dmx_batch_create",
@@ -96,6 +96,7 @@ def lib(path: str | None = None):
     L.dmx_profile.argtypes = [vp, ctypes.c_int32, dp, dp, ctypes.c_int32]
     L.dmx_kernel_name.argtypes = [ctypes.c_int32]
     L.dmx_tune.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int32)]
+    L.dmx_set_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
     fp = ctypes.POINTER(ctypes.c_float)
     L.dmx_set_limiter.argtypes = [vp, fp, ctypes.c_int32]
     L.dmx_learned_slopes.argtypes = [vp, fp, dp]
@@ -232,6 +233,14 @@ class Domain:
         self.shape = (int(sh[0]), int(sh[1]))
         self.trace_shape = (int(sh[2]), int(sh[3]))
         return ms.value
+
+    def set_shape(self, flux, trace) -> None:
+        """Launch shapes without timing (dmx_set_shape): flux = (mode,
+        planes) of the face/edge stage, trace = (mode, planes)."""
+        sh = (ctypes.c_int32 * 4)(int(flux[0]), int(flux[1]), int(trace[0]), int(trace[1]))
+        _check(lib().dmx_set_shape(self.h, sh), "dmx_set_shape")
+        self.shape = (int(flux[0]), int(flux[1]))
+        self.trace_shape = (int(trace[0]), int(trace[1]))
 
     def set_limiter(self, theta) -> None:
         """Weights of the learned limiter (slope_type 4), 7304 floats."""

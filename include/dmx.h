@@ -179,6 +179,11 @@ const char* dmx_kernel_name(int32_t id);
  * reference's step: it takes the role of utils/pyutils/gpuconfig/gpuconfig.py,
  * which sweeps OpenACC vector sizes by re-running the code. */
 int  dmx_tune(dmx_ctx* ctx, double* best_ms, int32_t* shape);
+/* Sets the launch shapes without timing them: shape[4] as dmx_tune returns
+ * it (face/edge mode 0..2 with 4, 8, 16 or 32 planes, mode 2 not with 4;
+ * trace mode 0, or 1 with 1..64 planes); for runs that must repeat a shape
+ * tuned before.  No counterpart in the reference. */
+int  dmx_set_shape(dmx_ctx* ctx, const int32_t* shape);
 
 /* ---- batched ensemble -------------------------------------------------
  * Several independent contexts of one precision advanced together: every

@@ -75,6 +75,8 @@ def test_null_context_is_an_error():
     assert lib.dmx_write_dump(None, b"/nonexistent/x.bin", 0.0, 0.0) == -5
     t = ctypes.c_double()
     assert lib.dmx_read_dump(None, b"/nonexistent/x.bin", ctypes.byref(t), ctypes.byref(dt)) == -5
+    sh = (ctypes.c_int32 * 4)(0, 8, 0, 4)
+    assert lib.dmx_set_shape(None, sh) == -5
 
 
 @needs_lib

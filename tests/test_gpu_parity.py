@@ -228,3 +228,23 @@ def test_trace_z_march_bit_identical(kt, monkeypatch):
                 a, b = P.interior(out[0]), P.interior(out[1])
                 for v in range(8):
                     assert np.abs(a[v] - b[v]).max() <= 1e-5 * (np.abs(a[v]).max() + 1e-30), v
+
+
+def test_set_shape():
+    """dmx_set_shape applies a launch shape without timing: float64 gives
+    the bits of the default shape; invalid shapes are rejected."""
+    pb = P.random_smooth(20, 12, 14, seed=41)
+    print("[gpu] set_shape", flush=True)
+    out = []
+    for flux, trace in (((0, 8), (0, 4)), ((2, 16), (1, 5)), ((1, 4), (1, 32))):
+        d = abi.Domain(pb.params, pb.uin)
+        d.set_shape(flux, trace)
+        d.advance(3, 0.0)
+        out.append(d.download())
+        d.close()
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+    d = abi.Domain(pb.params, pb.uin)
+    for flux, trace in (((3, 8), (0, 4)), ((0, 5), (0, 4)), ((2, 4), (0, 4)), ((0, 8), (1, 0)), ((0, 8), (2, 4))):
+        with pytest.raises(abi.DmxError):
+            d.set_shape(flux, trace)
+    d.close()

@@ -86,3 +86,16 @@ def test_training_pairs_shapes():
     assert X.shape == (3 * 8 ** 3, 40) and T.shape == (3 * 8 ** 3, 8)
     assert np.isfinite(X).all() and np.isfinite(T).all()
     assert np.abs(X).max() <= 1.0 + 1e-6
+
+
+def test_tvd_projection_is_accepted_by_the_rule():
+    """Targets projected by tvd_project pass the limiter's TVD test (away
+    from extrema) and are zero at extrema, where van Leer is zero."""
+    g = np.random.Generator(np.random.PCG64(5))
+    dl, dr, t = g.normal(size=(3, 10000))
+    p = L.tvd_project(t, dl, dr)
+    mono = dl * dr > 0
+    assert np.all((p[mono] * dl[mono] > 0) & (np.abs(p[mono]) <= 2.0 * np.minimum(np.abs(dl[mono]), np.abs(dr[mono]))))
+    assert np.all(p[~mono] == 0.0)
+    ok = mono & (t * dl > 0) & (np.abs(t) <= 2.0 * np.minimum(np.abs(dl), np.abs(dr)))
+    assert np.array_equal(p[ok], t[ok])
