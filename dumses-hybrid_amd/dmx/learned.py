@@ -242,21 +242,21 @@ def stencils(u_coarse: np.ndarray, gamma: float, d: int) -> np.ndarray:
     return st.reshape(8, 5, -1).transpose(2, 1, 0)
 
 
-def tvd_project(t, dl, dr):
+def tvd_project(t, dl, dr, margin: float = 1.0):
     """The slope t moved into the region the limiter rule accepts: at an
     extremum (dl dr <= 0) the van Leer slope (zero); elsewhere the sign of
-    dl and at most twice the smaller of |dl|, |dr| (a slope of the wrong
-    sign becomes 1e-3 of that bound)."""
-    bound = 2.0 * np.minimum(np.abs(dl), np.abs(dr))
+    dl and at most `margin` times twice the smaller of |dl|, |dr| (a slope
+    of the wrong sign becomes 1e-3 of that bound)."""
+    bound = margin * 2.0 * np.minimum(np.abs(dl), np.abs(dr))
     sg = np.sign(dl)
     mag = np.where(t * dl > 0, np.minimum(np.abs(t), bound), 1e-3 * bound)
     return np.where(dl * dr > 0, sg * mag, 0.0)
 
 
-def training_pairs(u_fine: np.ndarray, gamma: float, tvd: bool = False):
+def training_pairs(u_fine: np.ndarray, gamma: float, tvd: bool = False, margin: float = 1.0):
     """Features X (n, 40) and normalised targets T (n, 8) of every coarse
     cell and direction of one fine state; with tvd, the targets projected
-    into the region the limiter rule accepts (tvd_project)."""
+    into the region the limiter rule accepts (tvd_project, with margin)."""
     uc = restrict_state(u_fine)
     tgt = target_slopes(u_fine, gamma)
     Xs, Ts = [], []
@@ -265,7 +265,7 @@ def training_pairs(u_fine: np.ndarray, gamma: float, tvd: bool = False):
         X, s = features(q5)
         t = tgt[d].reshape(8, -1).T
         if tvd:
-            t = tvd_project(t, q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2])
+            t = tvd_project(t, q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2], margin)
         Ts.append(np.clip(np.where(s > 0, t / np.where(s > 0, s, 1.0), 0.0), -4.0, 4.0))
         Xs.append(X)
     return np.concatenate(Xs).astype(np.float32), np.concatenate(Ts).astype(np.float32)

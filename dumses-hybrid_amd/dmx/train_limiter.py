@@ -76,12 +76,13 @@ def run_ensemble(boxes, nsnap: int = 5, every: int = 6):
     return [(u, boxes[b].params.gamma) for b in range(len(boxes)) for u in snaps[b]]
 
 
-def train(out: str = WEIGHTS, steps: int = 3000, batch: int = 4096, seed: int = 0, tvd: bool = False, log=print):
+def train(out: str = WEIGHTS, steps: int = 3000, batch: int = 4096, seed: int = 0, tvd: bool = False,
+          margin: float = 1.0, log=print):
     t0 = time.perf_counter()
     boxes = ensemble_boxes()
     snaps = run_ensemble(boxes)
     t1 = time.perf_counter()
-    Xs, Ts = zip(*(L.training_pairs(u, gam, tvd=tvd) for u, gam in snaps))
+    Xs, Ts = zip(*(L.training_pairs(u, gam, tvd=tvd, margin=margin) for u, gam in snaps))
     X, T = np.concatenate(Xs), np.concatenate(Ts)
     log(f"[train] {len(boxes)} boxes, {len(snaps)} snapshots, {X.shape[0]} samples "
         f"(ensemble {t1 - t0:.1f} s, pairs {time.perf_counter() - t1:.1f} s)")
@@ -106,7 +107,8 @@ def train(out: str = WEIGHTS, steps: int = 3000, batch: int = 4096, seed: int = 
     base = 0.5 * ((mc - Tv) ** 2).sum() / Tv.shape[0]
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     np.savez(out, theta=theta, losses=np.concatenate(losses), val_loss=np.float64(vl),
-             zero_loss=np.float64(base), samples=np.int64(X.shape[0]), tvd_targets=np.bool_(tvd))
+             zero_loss=np.float64(base), samples=np.int64(X.shape[0]), tvd_targets=np.bool_(tvd),
+             tvd_margin=np.float64(margin))
     log(f"[train] wrote {out}: validation loss {vl:.4e} (zero-slope loss {base:.4e})")
     m.close()
     return theta
@@ -187,9 +189,11 @@ def main(argv=None):
     ap.add_argument("--json", default=None, help="also write the result to this file")
     ap.add_argument("--tvd-targets", action="store_true",
                     help="train on targets projected into the limiter's TVD region")
+    ap.add_argument("--tvd-margin", type=float, default=1.0,
+                    help="fraction of the TVD bound the projected targets keep to")
     a = ap.parse_args(argv)
     if a.cmd == "train":
-        train(a.out, a.steps, tvd=a.tvd_targets)
+        train(a.out, a.steps, tvd=a.tvd_targets, margin=a.tvd_margin)
     else:
         theta = np.load(a.weights)["theta"]
         res = evaluate(theta, a.n, tend=a.tend, fine_precision=a.fine_precision, kmax=a.kmax)

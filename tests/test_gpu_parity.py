@@ -248,3 +248,23 @@ def test_set_shape():
         with pytest.raises(abi.DmxError):
             d.set_shape(flux, trace)
     d.close()
+
+
+def test_fp32_shearing_box_five_orbits_diagnostics():
+    """The shearing-box MRI with FARGO at 128 x 256 x 128 over 5 orbits,
+    float32 bulk against float64, both landing on every orbit: the
+    box-averaged magnetic energy and Maxwell and Reynolds stresses agree to
+    1e-3 relative at every orbit while the instability grows them by five
+    orders of magnitude; the pointwise difference, which the growth
+    amplifies, stays below 1e-5 over the first orbit and 1e-3 over five."""
+    from dmx import shear_precision as SP
+    print("[gpu] shearing box fp32 vs fp64, 5 orbits", flush=True)
+    res = SP.run(5)
+    rows = res["orbits"]
+    assert rows[0]["l1_max"] < 1e-5
+    assert rows[-1]["l1_max"] < 1e-3
+    assert rows[-1]["maxwell"][0] > 1e4 * rows[0]["maxwell"][0]
+    for r in rows:
+        for key in ("magp", "maxwell", "reynolds"):
+            a, b = r[key]
+            assert abs(b - a) <= 1e-3 * abs(a), (r["orbit"], key, a, b)
