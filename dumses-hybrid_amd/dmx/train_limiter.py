@@ -4,9 +4,9 @@
     python -m dmx.train_limiter eval  [--weights PATH] [--n 32]
 
 train: an ensemble of decaying-turbulence boxes of uneven sizes is run at
-fine resolution by the plain solver (float32 bulk, HLLD/HLLD, MC slopes),
-one context and HIP stream per box, all boxes advancing concurrently from
-host threads; snapshots of each fine state give coarse/fine training pairs
+fine resolution by the plain solver (float32 bulk, HLLD/HLLD, MC slopes) as
+one batch (dmx_batch_*, every kernel one launch over all boxes); snapshots
+of each fine state give coarse/fine training pairs
 (dmx.learned.training_pairs), and the network is trained on the GPU by the
 library's own forward/backward/Adam kernels (dmx_mlp_*).
 
@@ -21,7 +21,6 @@ from __future__ import annotations
 import argparse
 import os
 import sys
-import threading
 import time
 
 import numpy as np
@@ -50,29 +49,19 @@ def ensemble_boxes(nboxes: int = 12, seed: int = 4242):
 
 
 def run_ensemble(boxes, nsnap: int = 5, every: int = 6):
-    """Advances every box concurrently (one context and stream each, one host
-    thread each) and returns the fine snapshots [(uin, gamma)]."""
+    """Advances every box in one batch (dmx_batch_*: each kernel of the step
+    one launch over the tiles of all boxes) and returns the fine snapshots
+    [(uin, gamma)], box by box."""
+    doms = [abi.Domain(pb.params, pb.uin) for pb in boxes]
+    batch = abi.Batch(doms)
     snaps = [[] for _ in boxes]
-    errors = []
-
-    def work(b, pb):
-        try:
-            d = abi.Domain(pb.params, pb.uin)
-            t = 0.0
-            for _ in range(nsnap):
-                t, _ = d.advance(every, t)
-                snaps[b].append(d.download())
-            d.close()
-        except Exception as e:   # reported after the join
-            errors.append(e)
-
-    threads = [threading.Thread(target=work, args=(b, pb)) for b, pb in enumerate(boxes)]
-    for th in threads:
-        th.start()
-    for th in threads:
-        th.join()
-    if errors:
-        raise errors[0]
+    for _ in range(nsnap):
+        batch.advance(every)
+        for b, d in enumerate(doms):
+            snaps[b].append(d.download())
+    batch.close()
+    for d in doms:
+        d.close()
     return [(u, boxes[b].params.gamma) for b in range(len(boxes)) for u in snaps[b]]
 
 
