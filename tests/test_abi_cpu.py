@@ -93,3 +93,55 @@ def test_batch_rejects_missing_contexts():
     ms = ctypes.c_double()
     assert lib.dmx_batch_last_ms(None, ctypes.byref(ms)) == -5
     lib.dmx_batch_destroy(None)
+
+
+@needs_lib
+def test_c_usage_compiles_and_links(tmp_path):
+    """The C usage INTEGRATION.md documents (context, upload, advance,
+    download, autotuner, batch) compiles against include/dmx.h with a plain
+    C compiler and links against libdmx.so (not run: no GPU here)."""
+    src = tmp_path / "use.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stdlib.h>
+#include "dmx.h"
+int main(void)
+{
+    dmx_params p = { .nx = 64, .ny = 64, .nz = 64, .iso = 0, .slope_type = 2,
+                     .riemann = DMX_HLLD, .riemann2d = DMX_HLLD,
+                     .bdtype = {DMX_BC_PERIODIC, DMX_BC_PERIODIC, DMX_BC_PERIODIC},
+                     .fargo = 0, .precision = DMX_FP64,
+                     .xmin = 0, .xmax = 1, .ymin = 0, .ymax = 1, .zmin = 0, .zmax = 1,
+                     .gamma = 5.0 / 3.0, .ciso = 0, .omega0 = 0, .courant = 0.7,
+                     .nu = 0, .eta = 0 };
+    dmx_ctx* ctx[2];
+    size_t n = (size_t)70 * 70 * 70 * 11;
+    double* uin = calloc(n, sizeof(double));
+    for (int b = 0; b < 2; b++) {
+        if (dmx_create(&p, &ctx[b])) { fprintf(stderr, "%s\n", dmx_last_error()); return 1; }
+        dmx_upload(ctx[b], uin);
+    }
+    double t = 0.0, dts[10], ms;
+    int32_t shape[4];
+    dmx_tune(ctx[0], &ms, shape);
+    dmx_set_shape(ctx[0], shape);
+    dmx_advance(ctx[0], 10, &t, dts);
+    dmx_batch* bt;
+    double times[2] = {0.0, 0.0};
+    if (dmx_batch_create(ctx, 2, 0, &bt) == 0) {
+        dmx_batch_advance(bt, 5, times);
+        dmx_batch_destroy(bt);
+    }
+    dmx_download(ctx[0], uin);
+    for (int b = 0; b < 2; b++) dmx_destroy(ctx[b]);
+    free(uin);
+    return 0;
+}
+''')
+    import subprocess
+    libdir = os.path.dirname(abi.LIB_PATH)
+    exe = tmp_path / "use"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER), str(src),
+                        "-L", libdir, "-ldmx", f"-Wl,-rpath,{libdir}", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
