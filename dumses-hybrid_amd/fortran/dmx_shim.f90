@@ -73,6 +73,12 @@ module dmx_bind
        type(c_ptr), value :: ctx
        real(c_double), value :: time, dt
      end function dmx_dissipation
+     integer(c_int) function dmx_tune(ctx, best_ms, shape) bind(C, name='dmx_tune')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: best_ms
+       integer(c_int32_t), intent(out) :: shape(4)
+     end function dmx_tune
      type(c_ptr) function dmx_last_error() bind(C, name='dmx_last_error')
        import :: c_ptr
      end function dmx_last_error
@@ -141,14 +147,26 @@ contains
   end subroutine dmx_ensure_ctx
 
   ! hand the host state to the device (start of the time loop, after init
-  ! and restart_run may have rewritten uin)
+  ! and restart_run may have rewritten uin); on the first upload the launch
+  ! shapes are autotuned on that state (it is left unchanged; float64
+  ! results do not depend on the shape) unless DMX_TUNE=0
   subroutine dmx_to_device()
     use variables, only: uin
+    logical, save :: tuned = .false.
+    character(len=8) :: tv
+    integer :: tlen, tstat
+    real(c_double) :: ms
+    integer(c_int32_t) :: shape(4)
     if (on_device) return
     call dmx_ensure_ctx()
     call dmx_check(dmx_upload(ctx, uin), 'dmx_upload')
     on_device = .true.
     host_stale = .false.
+    if (.not. tuned) then
+       tuned = .true.
+       call get_environment_variable('DMX_TUNE', tv, tlen, tstat)
+       if (.not. (tstat == 0 .and. tv(1:1) == '0')) call dmx_check(dmx_tune(ctx, ms, shape), 'dmx_tune')
+    end if
   end subroutine dmx_to_device
 
   ! bring the device state back before host code reads uin
