@@ -59,35 +59,6 @@ __device__ __forceinline__ float van_leer(float dl, float dr)
     return p > 0.0f ? 2.0f * p / (dl + dr) : 0.0f;
 }
 
-// The limiter rule on the network slopes y[8] of a cell whose stencil
-// centre values are q0[8] with one-sided differences dl, dr: a variable
-// whose slope leaves the TVD region (sign of dl and dr, at most twice the
-// smaller of |dl|, |dr|, zero at an extremum) takes van Leer; if density or
-// pressure would be non-positive at a face, all 8 take van Leer.  Returns
-// true when the cell fell back; *nrep counts the variables that the TVD
-// rule replaced.
-__device__ __forceinline__ bool limit_slopes(float y[8], const float q0[8], const float dl[8], const float dr[8],
-                                             unsigned int* nrep)
-{
-    float vl[8];
-    unsigned int r = 0;
-#pragma unroll
-    for (int v = 0; v < 8; v++) {
-        vl[v] = van_leer(dl[v], dr[v]);
-        bool tvd = (dl[v] * dr[v] > 0.0f) && (y[v] * dl[v] > 0.0f)
-                   && (fabsf(y[v]) <= 2.0f * fminf(fabsf(dl[v]), fabsf(dr[v])));
-        y[v] = tvd ? y[v] : vl[v];
-        r += tvd ? 0u : 1u;
-    }
-    *nrep = r;
-    bool pos = (q0[0] - 0.5f * fabsf(y[0]) > 0.0f) && (q0[4] - 0.5f * fabsf(y[4]) > 0.0f);
-    if (!pos) {
-#pragma unroll
-        for (int v = 0; v < 8; v++) y[v] = vl[v];
-    }
-    return !pos;
-}
-
 // ---------------------------------------------------------------------------
 // bf16 MFMA (gfx950 v_mfma_f32_16x16x32_bf16): D[16x16] += A[16x32] B[32x16].
 // Lane l holds A row (l % 16), k = 8 (l / 16) + 0..7; B column (l % 16),

@@ -134,17 +134,19 @@ def van_leer(dl, dr):
 
 
 def limited_slopes(theta, q5):
-    """The device rule: the network's slopes, except that a variable whose
+    """The device rule: the van Leer slope plus the network's correction
+    (its output times the stencil scale), except that a variable whose
     slope leaves the TVD region (sign of the one-sided differences, at most
     twice the smaller one, zero at an extremum) takes the van Leer slope, and
     a cell whose density or pressure would turn non-positive at a face takes
-    van Leer for all 8 variables.  Returns (slopes (n, 8), fell_back (n,),
-    replaced (n, 8): the variables the TVD rule replaced)."""
+    van Leer for all 8 variables.  A network that outputs zero is the van
+    Leer limiter.  Returns (slopes (n, 8), fell_back (n,), replaced (n, 8):
+    the variables the TVD rule replaced)."""
     X, s = features(q5)
-    y = forward(theta, X) * s
     dl = q5[:, 2] - q5[:, 1]
     dr = q5[:, 3] - q5[:, 2]
     vl = van_leer(dl, dr)
+    y = vl + forward(theta, X) * s
     tvd = (dl * dr > 0) & (y * dl > 0) & (np.abs(y) <= 2.0 * np.minimum(np.abs(dl), np.abs(dr)))
     out = np.where(tvd, y, vl)
     q0 = q5[:, 2]
@@ -255,8 +257,10 @@ def tvd_project(t, dl, dr, margin: float = 1.0):
 
 def training_pairs(u_fine: np.ndarray, gamma: float, tvd: bool = False, margin: float = 1.0):
     """Features X (n, 40) and normalised targets T (n, 8) of every coarse
-    cell and direction of one fine state; with tvd, the targets projected
-    into the region the limiter rule accepts (tvd_project, with margin)."""
+    cell and direction of one fine state: the difference between the true
+    slope and the van Leer slope over the stencil scale, clipped to [-4, 4];
+    with tvd, the true slope first projected into the region the limiter
+    rule accepts (tvd_project, with margin)."""
     uc = restrict_state(u_fine)
     tgt = target_slopes(u_fine, gamma)
     Xs, Ts = [], []
@@ -264,8 +268,10 @@ def training_pairs(u_fine: np.ndarray, gamma: float, tvd: bool = False, margin: 
         q5 = stencils(uc, gamma, d)
         X, s = features(q5)
         t = tgt[d].reshape(8, -1).T
+        dl, dr = q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2]
         if tvd:
-            t = tvd_project(t, q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2], margin)
+            t = tvd_project(t, dl, dr, margin)
+        t = t - van_leer(dl, dr)
         Ts.append(np.clip(np.where(s > 0, t / np.where(s > 0, s, 1.0), 0.0), -4.0, 4.0))
         Xs.append(X)
     return np.concatenate(Xs).astype(np.float32), np.concatenate(Ts).astype(np.float32)

@@ -1,7 +1,10 @@
 """Training and evaluation of the learned slope limiter.
 
-    python -m dmx.train_limiter train [--out PATH] [--steps N]
-    python -m dmx.train_limiter eval  [--weights PATH] [--n 32]
+    python -m dmx.train_limiter train [--out PATH] [--steps N] [--boxes B] [--sizes 32,64,96,128]
+    python -m dmx.train_limiter eval  [--weights PATH] [--n 32] [--tend T] [--kmax K]
+
+The defaults of train are the recipe of the shipped weights
+(data/limiter_weights.npz).
 
 train: an ensemble of decaying-turbulence boxes of uneven sizes is run at
 fine resolution by the plain solver (float32 bulk, HLLD/HLLD, MC slopes) as
@@ -14,7 +17,9 @@ eval: from one fine initial state, runs (a) the plain solver at twice the
 resolution, restricted to the coarse grid (the reference), (b) the plain
 solver (MC) on the coarse grid and (c) the learned limiter on the coarse
 grid, to the same time, and reports the L1 difference of (b) and (c) from
-(a) per variable, with the wall time of each run.
+(a) per variable, with the wall time of each run.  The network corrects
+the van Leer slopes (dmx.learned.limited_slopes): its targets are the
+sub-cell slope minus van Leer, over the stencil scale.
 """
 from __future__ import annotations
 
@@ -33,14 +38,14 @@ DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 WEIGHTS = os.path.join(DATA, "limiter_weights.npz")
 
 
-def ensemble_boxes(nboxes: int = 12, seed: int = 4242):
-    """Fine sizes drawn from {32, 48, 64}^3 (coarse 16..32), random-phase
-    Gaussian v and A (k^-2, k <= 8 in coarse-grid units), Mach and beta
+def ensemble_boxes(nboxes: int = 12, seed: int = 4242, sizes=(32, 48, 64)):
+    """Fine sizes drawn from `sizes` (cubes; coarse half that), random-phase
+    Gaussian v and A (k^-2, k <= n/8 in fine-grid units), Mach and beta
     drawn from {0.5, 1} x {1, 10}."""
     g = np.random.Generator(np.random.PCG64(seed))
     boxes = []
     for b in range(nboxes):
-        n = int(g.choice([32, 48, 64]))
+        n = int(g.choice(list(sizes)))
         mach = float(g.choice([0.5, 1.0]))
         beta = float(g.choice([1.0, 10.0]))
         boxes.append(P.turbulence(n, seed=seed + b, mach=mach, beta=beta, kmax=max(4, n // 8),
@@ -65,15 +70,35 @@ def run_ensemble(boxes, nsnap: int = 5, every: int = 6):
     return [(u, boxes[b].params.gamma) for b in range(len(boxes)) for u in snaps[b]]
 
 
-def train(out: str = WEIGHTS, steps: int = 3000, batch: int = 4096, seed: int = 0, tvd: bool = False,
-          margin: float = 1.0, log=print):
+def train(out: str = WEIGHTS, steps: int = 6000, batch: int = 4096, seed: int = 0, tvd: bool = True,
+          margin: float = 1.0, nboxes: int = 24, sizes=(32, 64, 96, 128), per_snapshot: int = 60000,
+          log=print):
+    """per_snapshot > 0 keeps that many samples of each snapshot, drawn
+    at random (bounds the memory of large ensembles)."""
     t0 = time.perf_counter()
-    boxes = ensemble_boxes()
+    boxes = ensemble_boxes(nboxes, sizes=sizes)
     snaps = run_ensemble(boxes)
     t1 = time.perf_counter()
-    Xs, Ts = zip(*(L.training_pairs(u, gam, tvd=tvd, margin=margin) for u, gam in snaps))
+    gs = np.random.Generator(np.random.PCG64(seed + 99))
+
+    def pairs(u, gam):
+        X, T = L.training_pairs(u, gam, tvd=tvd, margin=margin)
+        if per_snapshot and X.shape[0] > per_snapshot:
+            keep = gs.choice(X.shape[0], per_snapshot, replace=False)
+            X, T = X[keep], T[keep]
+        return X, T
+
+    nsnap = len(snaps)
+    Xs, Ts = [], []
+    while snaps:
+        X, T = pairs(*snaps.pop(0))
+        Xs.append(X)
+        Ts.append(T)
+        if len(Xs) % 10 == 0:
+            log(f"[train] pairs from {len(Xs)} of {nsnap} snapshots")
     X, T = np.concatenate(Xs), np.concatenate(Ts)
-    log(f"[train] {len(boxes)} boxes, {len(snaps)} snapshots, {X.shape[0]} samples "
+    del Xs, Ts
+    log(f"[train] {len(boxes)} boxes, {nsnap} snapshots, {X.shape[0]} samples "
         f"(ensemble {t1 - t0:.1f} s, pairs {time.perf_counter() - t1:.1f} s)")
     g = np.random.Generator(np.random.PCG64(seed))
     perm = g.permutation(X.shape[0])
@@ -85,20 +110,21 @@ def train(out: str = WEIGHTS, steps: int = 3000, batch: int = 4096, seed: int = 
     m.load_data(Xt, Tt)
     losses = []
     for phase, (n, lr) in enumerate(((steps * 2 // 3, 2e-3), (steps - steps * 2 // 3, 5e-4))):
+        tp = time.perf_counter()
         ls = m.train(n, batch, lr, seed=seed + 17 * phase)
         losses.append(ls)
         vl, _ = m.loss_grad(Xv, Tv)
-        log(f"[train] phase {phase}: {n} steps lr {lr:g}, last minibatch loss "
-            f"{ls[-50:].mean():.4e}, validation loss {vl:.4e}")
+        log(f"[train] phase {phase}: {n} steps lr {lr:g} in {time.perf_counter() - tp:.1f} s, last minibatch "
+            f"loss {ls[-50:].mean():.4e}, validation loss {vl:.4e}")
     theta = m.get_params()
-    # the loss of predicting MC slopes, for scale
+    # the loss of a zero network (the van Leer slopes), for scale
     mc = np.zeros_like(Tv)
     base = 0.5 * ((mc - Tv) ** 2).sum() / Tv.shape[0]
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     np.savez(out, theta=theta, losses=np.concatenate(losses), val_loss=np.float64(vl),
              zero_loss=np.float64(base), samples=np.int64(X.shape[0]), tvd_targets=np.bool_(tvd),
              tvd_margin=np.float64(margin))
-    log(f"[train] wrote {out}: validation loss {vl:.4e} (zero-slope loss {base:.4e})")
+    log(f"[train] wrote {out}: validation loss {vl:.4e} (zero-network loss {base:.4e})")
     m.close()
     return theta
 
@@ -170,19 +196,23 @@ def main(argv=None):
     ap.add_argument("cmd", choices=("train", "eval"))
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--weights", default=WEIGHTS)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--tend", type=float, default=0.25)
     ap.add_argument("--fine-precision", type=int, default=1, help="0: float64 fine run, 1: float32 bulk")
     ap.add_argument("--kmax", type=int, default=None)
     ap.add_argument("--json", default=None, help="also write the result to this file")
-    ap.add_argument("--tvd-targets", action="store_true",
-                    help="train on targets projected into the limiter's TVD region")
+    ap.add_argument("--no-tvd-targets", dest="tvd_targets", action="store_false",
+                    help="train on unprojected targets (default: projected into the limiter's TVD region)")
     ap.add_argument("--tvd-margin", type=float, default=1.0,
                     help="fraction of the TVD bound the projected targets keep to")
+    ap.add_argument("--boxes", type=int, default=24)
+    ap.add_argument("--sizes", default="32,64,96,128", help="fine box sizes to draw from")
+    ap.add_argument("--per-snapshot", type=int, default=60000, help="samples kept per snapshot (0: all)")
     a = ap.parse_args(argv)
     if a.cmd == "train":
-        train(a.out, a.steps, tvd=a.tvd_targets, margin=a.tvd_margin)
+        train(a.out, a.steps, tvd=a.tvd_targets, margin=a.tvd_margin, nboxes=a.boxes,
+              sizes=tuple(int(x) for x in a.sizes.split(",")), per_snapshot=a.per_snapshot)
     else:
         theta = np.load(a.weights)["theta"]
         res = evaluate(theta, a.n, tend=a.tend, fine_precision=a.fine_precision, kmax=a.kmax)

@@ -101,6 +101,26 @@ def test_limiter_kernel_matches_reference():
     assert abs(frep - frep_ref) < 0.05
 
 
+def test_zero_network_is_van_leer():
+    """A network that outputs zero leaves the van Leer slope of the float
+    primitives, which the rule always accepts."""
+    pb = P.turbulence(16, seed=13, mach=1.0, beta=1.0)
+    p = pb.params.replace(slope_type=4, precision=1)
+    print("[gpu] zero network", flush=True)
+    d = abi.Domain(p, pb.uin)
+    d.set_limiter(np.zeros(L.NPARAM, np.float32))
+    got, fb, frep = d.learned_slopes()
+    u = d.download()
+    d.close()
+    for dd in range(3):
+        q5 = L.stencils(u, p.gamma, dd).astype(np.float32).astype(np.float64)
+        vl = L.van_leer(q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2]).reshape(p.nz, p.ny, p.nx, 8).transpose(3, 0, 1, 2)
+        _, s = L.features(q5)
+        scale = s.reshape(p.nz, p.ny, p.nx, 8).transpose(3, 0, 1, 2)
+        assert np.all(np.abs(got[dd] - vl) <= 1e-5 * scale + 1e-30)
+    assert fb == 0.0
+
+
 def test_learned_step_conserves_and_keeps_div_b():
     pb = P.turbulence(16, seed=12, mach=0.5, beta=10.0)
     p = pb.params.replace(slope_type=4, precision=1)

@@ -88,6 +88,30 @@ def test_training_pairs_shapes():
     assert np.abs(X).max() <= 1.0 + 1e-6
 
 
+def test_targets_are_corrections_to_van_leer():
+    """van Leer + target x scale gives back the (projected) sub-cell slope
+    of target_slopes wherever the target is not clipped: a network that
+    fits the targets reproduces the fine slopes through the limiter rule."""
+    pb = P.turbulence(16, seed=8, mach=1.0, beta=1.0)
+    gam = pb.params.gamma
+    uc = L.restrict_state(pb.uin)
+    tgt = L.target_slopes(pb.uin, gam)
+    for tvd in (False, True):
+        X, T = L.training_pairs(pb.uin, gam, tvd=tvd)
+        n = T.shape[0] // 3
+        for d in range(3):
+            q5 = L.stencils(uc, gam, d)
+            _, s = L.features(q5)
+            dl, dr = q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2]
+            t = tgt[d].reshape(8, -1).T
+            if tvd:
+                t = L.tvd_project(t, dl, dr)
+            rec = L.van_leer(dl, dr) + T[d * n:(d + 1) * n].astype(np.float64) * s
+            keep = np.abs(T[d * n:(d + 1) * n]) < 4.0
+            assert keep.mean() > 0.95
+            assert np.allclose(rec[keep], t[keep], rtol=0, atol=1e-6 * np.broadcast_to(s, t.shape)[keep].max())
+
+
 def test_tvd_projection_is_accepted_by_the_rule():
     """Targets projected by tvd_project pass the limiter's TVD test (away
     from extrema) and are zero at extrema, where van Leer is zero."""
