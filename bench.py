@@ -98,7 +98,7 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
 # planes), (trace mode, planes)); --no-tune applies them without timing, so
 # that profiler runs measure the kernels of the timed run (under a profiler
 # the autotuner's timings, and so its choice, can change)
-TUNED_256 = {0: ((0, 8), (1, 16)), 1: ((2, 8), (1, 16))}
+TUNED_256 = {0: ((0, 8), (1, 16)), 1: ((2, 8), (1, 8))}
 
 
 def run_gpu(pb, steps, warmup, precision, limiter=None, tune=True):
