@@ -57,12 +57,12 @@ def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
 # profiles/mhd_step_ot256_hbm_bytes.md (FETCH_SIZE scaled by 2.00, WRITE_SIZE
 # by 1.00 against the known-size calibration kernels of the same runs), for
 # the launch shapes the autotuner keeps at 256^3 (float64: fused flux kernel,
-# 8 planes; float32: carrying flux kernel, 8 planes; z-marching trace, 16
-# planes)
+# 8 planes; float32: carrying flux kernel, 8 planes; z-marching trace on a
+# 32x8 column tile, 16 planes in float64 and 8 in float32)
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 638.9, "flux": 1651.6, "update_ct": 368.9, "compute_dt": 85.9,
+    "f64": {"trace": 570.7, "flux": 1649.2, "update_ct": 368.9, "compute_dt": 86.0,
             "boundary": 17.5},
-    "f32": {"trace": 393.6, "flux": 837.5, "update_ct": 248.2, "compute_dt": 64.8,
+    "f32": {"trace": 348.3, "flux": 837.4, "update_ct": 248.1, "compute_dt": 64.8,
             "boundary": 15.1},
 }
 
@@ -72,8 +72,8 @@ MEASURED_TRAFFIC = {
 # in profiles/mhd_step_ot256_sq_counters.md: the bound that applies where the
 # HBM fraction is low (bit-exact float64 division and square root)
 MEASURED_VALU_BUSY = {
-    "f64": {"trace": 0.407, "flux": 0.701, "update_ct": 0.130, "compute_dt": 0.783, "boundary": 0.118},
-    "f32": {"trace": 0.325, "flux": 0.564, "update_ct": 0.128, "compute_dt": 0.442, "boundary": 0.158},
+    "f64": {"trace": 0.451, "flux": 0.705, "update_ct": 0.131, "compute_dt": 0.777, "boundary": 0.119},
+    "f32": {"trace": 0.404, "flux": 0.566, "update_ct": 0.129, "compute_dt": 0.454, "boundary": 0.158},
 }
 
 
