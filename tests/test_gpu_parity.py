@@ -207,7 +207,8 @@ def test_trace_z_march_bit_identical(kt, monkeypatch):
     cases = [P.random_smooth(21, 13, 10, seed=31, slope_type=3),
              P.random_smooth(17, 9, 23, seed=32, slope_type=2,
                              bdtype=(P.ZERO_GRADIENT, P.PERIODIC, P.ZERO_GRADIENT)),
-             P.mri_test(16, 32, 16)]
+             P.mri_test(16, 32, 16),
+             P.random_smooth(45, 11, 7, seed=33, slope_type=2)]   # two 32-wide x tiles
     for prec in (0, 1):
         for pb in cases:
             p = pb.params.replace(precision=prec)
