@@ -231,6 +231,62 @@ def test_trace_z_march_bit_identical(kt, monkeypatch):
                     assert np.abs(a[v] - b[v]).max() <= 1e-5 * (np.abs(a[v]).max() + 1e-30), v
 
 
+def test_fused_update_dt_equals_separate_dt(monkeypatch):
+    """On periodic float64 domains advance and profile fuse the next step's
+    dt into the update (k_update_dt, faces alternating between two
+    buffers).  The states and dt sequences equal those of the separate dt
+    kernel (DMX_FUSE_DT=0) bit for bit, also when the run is split over
+    several calls (each call's last update is not fused and brings the
+    faces home); step-by-step calls and a batch then see the same state.
+    float32 (plain and learned limiter) does not fuse, so the switch leaves
+    it unchanged."""
+    from dmx import learned as L
+    cases = [(P.orszag_tang(16), 0), (P.random_smooth(21, 13, 10, seed=41, slope_type=2), 0),
+             (P.random_smooth(45, 11, 7, seed=42, slope_type=3), 0), (P.orszag_tang(16), 1)]
+    for pb, prec in cases:
+        p = pb.params.replace(precision=prec)
+        print(f"[gpu] fused dt {p.nx}x{p.ny}x{p.nz} precision {prec}", flush=True)
+        runs = []
+        for fuse in ("1", "0"):
+            monkeypatch.setenv("DMX_FUSE_DT", fuse)
+            d = abi.Domain(p, pb.uin)
+            t, dts_a = d.advance(4, 0.0)
+            t, dts_b = d.advance(3)
+            prof = d.profile(2)
+            assert prof["update_ct"] > 0.0
+            t, dts_c = d.advance(1)
+            runs.append((np.concatenate([dts_a, dts_b, dts_c]), d.download(), d))
+        (dts1, u1, d1), (dts0, u0, d0) = runs
+        assert np.array_equal(dts1, dts0)
+        assert np.array_equal(u1, u0)
+        if prec == 0:
+            # the state a step-by-step call and a batch see is the same
+            for d in (d1, d0):
+                d.godunov(d.time, d.compute_dt())
+                d.boundary(d.time, 0.0)
+            assert np.array_equal(d1.download(), d0.download())
+            for d in (d1, d0):
+                b = abi.Batch([d])
+                b.advance(2)
+                b.close()
+            assert np.array_equal(d1.download(), d0.download())
+        d1.close()
+        d0.close()
+    # the learned limiter (float32, slope_type 4) through the fused path
+    pb = P.orszag_tang(16)
+    p = pb.params.replace(precision=1, slope_type=4)
+    out = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("DMX_FUSE_DT", fuse)
+        d = abi.Domain(p, pb.uin)
+        d.set_limiter(0.5 * L.init_params(9))
+        _, dts = d.advance(5, 0.0)
+        out.append((dts, d.download()))
+        d.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_set_shape():
     """dmx_set_shape applies a launch shape without timing: float64 gives
     the bits of the default shape; invalid shapes are rejected."""
