@@ -58,12 +58,15 @@ def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
 # by 1.00 against the known-size calibration kernels of the same runs), for
 # the launch shapes the autotuner keeps at 256^3 (float64: fused flux kernel,
 # 8 planes; float32: carrying flux kernel, 8 planes; z-marching trace on a
-# 32x8 column tile, 16 planes in float64 and 8 in float32)
+# 32x8 column tile, 16 planes in float64 and 8 in float32).  The float64
+# update is k_update_dt (the next dt fused in on periodic domains), whose
+# compute_dt group is then k_dt_final alone after the first step; the
+# compute_dt figures are those of k_dt_min.
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 570.7, "flux": 1649.2, "update_ct": 368.9, "compute_dt": 86.0,
-            "boundary": 17.5},
-    "f32": {"trace": 348.3, "flux": 837.4, "update_ct": 248.1, "compute_dt": 64.8,
-            "boundary": 15.1},
+    "f64": {"trace": 571.0, "flux": 1653.1, "update_ct": 416.8, "compute_dt": 86.0,
+            "boundary": 17.7},
+    "f32": {"trace": 348.2, "flux": 837.6, "update_ct": 248.2, "compute_dt": 64.8,
+            "boundary": 14.1},
 }
 
 
@@ -72,8 +75,8 @@ MEASURED_TRAFFIC = {
 # in profiles/mhd_step_ot256_sq_counters.md: the bound that applies where the
 # HBM fraction is low (bit-exact float64 division and square root)
 MEASURED_VALU_BUSY = {
-    "f64": {"trace": 0.451, "flux": 0.705, "update_ct": 0.131, "compute_dt": 0.777, "boundary": 0.119},
-    "f32": {"trace": 0.404, "flux": 0.566, "update_ct": 0.129, "compute_dt": 0.454, "boundary": 0.158},
+    "f64": {"trace": 0.449, "flux": 0.699, "update_ct": 0.384, "compute_dt": 0.704, "boundary": 0.046},
+    "f32": {"trace": 0.405, "flux": 0.562, "update_ct": 0.126, "compute_dt": 0.414, "boundary": 0.055},
 }
 
 
