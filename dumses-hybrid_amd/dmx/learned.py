@@ -135,20 +135,23 @@ def van_leer(dl, dr):
 
 def limited_slopes(theta, q5):
     """The device rule: the van Leer slope plus the network's correction
-    (its output times the stencil scale), except that a variable whose
-    slope leaves the TVD region (sign of the one-sided differences, at most
-    twice the smaller one, zero at an extremum) takes the van Leer slope, and
-    a cell whose density or pressure would turn non-positive at a face takes
-    van Leer for all 8 variables.  A network that outputs zero is the van
-    Leer limiter.  Returns (slopes (n, 8), fell_back (n,), replaced (n, 8):
-    the variables the TVD rule replaced)."""
+    (its output times the stencil scale), kept in the TVD region: a
+    variable at an extremum or whose slope has the wrong sign takes the van
+    Leer slope, one larger than twice the smaller one-sided difference is
+    clamped to that bound; a cell whose density or pressure would turn
+    non-positive at a face takes van Leer for all 8 variables.  A network
+    that outputs zero is the van Leer limiter.  Returns (slopes (n, 8),
+    fell_back (n,), replaced (n, 8): the variables whose network slope left
+    the TVD region)."""
     X, s = features(q5)
     dl = q5[:, 2] - q5[:, 1]
     dr = q5[:, 3] - q5[:, 2]
     vl = van_leer(dl, dr)
     y = vl + forward(theta, X) * s
-    tvd = (dl * dr > 0) & (y * dl > 0) & (np.abs(y) <= 2.0 * np.minimum(np.abs(dl), np.abs(dr)))
-    out = np.where(tvd, y, vl)
+    sgn = (dl * dr > 0) & (y * dl > 0)
+    bound = 2.0 * np.minimum(np.abs(dl), np.abs(dr))
+    tvd = sgn & (np.abs(y) <= bound)
+    out = np.where(sgn, np.copysign(np.minimum(np.abs(y), bound), y), vl)
     q0 = q5[:, 2]
     pos = np.ones(q5.shape[0], dtype=bool)
     for v in (0, 4):

@@ -187,7 +187,7 @@ def evaluate(theta, n: int = 32, tend: float = 0.25, seed: int = 777, mach: floa
         d.close()
     log(f"[eval] n={n} fine {2 * n}: mean L1 mc {res['mc_l1_mean']:.4e}, learned "
         f"{res['learned_l1_mean']:.4e} (positivity fallback {res['learned_fallback']:.3f}, "
-        f"TVD-replaced slopes {res['learned_tvd_replaced']:.3f})")
+        f"network slopes outside the TVD region {res['learned_tvd_replaced']:.3f})")
     return res
 
 

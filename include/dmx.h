@@ -218,9 +218,10 @@ int  dmx_batch_last_ms(dmx_batch* batch, double* ms);
  * along one direction to corrections of the 8 van Leer slopes of the centre
  * cell (slope = van Leer + output x stencil scale, so a network that outputs
  * zero is the van Leer limiter); it runs on bf16 MFMA (float accumulation)
- * before the trace, and a slope that leaves the TVD region, or a cell whose
- * density or pressure would turn non-positive at a face, falls back to van
- * Leer.  Parameters are 7304 floats: W1[40][64], b1,
+ * before the trace.  A slope above the TVD bound (twice the smaller
+ * one-sided difference) is clamped to it; one of the wrong sign or at an
+ * extremum, or a cell whose density or pressure would turn non-positive at
+ * a face, falls back to van Leer.  Parameters are 7304 floats: W1[40][64], b1,
  * W2[64][64], b2, W3[64][8], b3 (row = input).  dumses-hybrid_amd/dmx/learned.py
  * is the float64 reference of features, network and fallback. */
 int  dmx_set_limiter(dmx_ctx* ctx, const float* theta, int32_t n);
@@ -228,7 +229,7 @@ int  dmx_set_limiter(dmx_ctx* ctx, const float* theta, int32_t n);
  * [direction][variable][k][j][i] (interior); fallback (may be NULL)
  * receives 2 values: the fraction of cell-directions that fell back to van
  * Leer for all 8 variables (positivity) and the fraction of single-variable
- * slopes that the TVD rule replaced by van Leer. */
+ * slopes that left the TVD region (clamped, or replaced by van Leer). */
 int  dmx_learned_slopes(dmx_ctx* ctx, float* out, double* fallback);
 
 /* Training of the network on the GPU (hand-written float32 forward and

@@ -94,7 +94,7 @@ def test_limiter_kernel_matches_reference():
         total += close.size
     # the reference counts interior cells, the device [0, n+1]^3
     frep_ref = rep_ref / total
-    print(f"[gpu] agreement {agree / total:.4f}, positivity fallback {fb:.4f}, TVD-replaced "
+    print(f"[gpu] agreement {agree / total:.4f}, positivity fallback {fb:.4f}, outside the TVD region "
           f"{frep:.4f} (reference {frep_ref:.4f})")
     assert agree / total > 0.98
     assert 0.0 <= fb <= 1.0

@@ -53,6 +53,31 @@ def test_zero_network_falls_back_to_van_leer():
     assert np.array_equal(out, vl)
 
 
+def test_rule_clamps_to_the_tvd_bound_and_falls_back():
+    """A network whose output is a constant c (only the last bias set):
+    where van Leer + c s has the sign of the one-sided differences it is
+    clamped to twice the smaller one; where it has the wrong sign, and at
+    extrema, the slope is van Leer."""
+    g = np.random.Generator(np.random.PCG64(8))
+    q5 = np.abs(g.standard_normal((4000, 5, 8))) + 0.5
+    dl, dr = q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2]
+    vl = L.van_leer(dl, dr)
+    bound = 2.0 * np.minimum(np.abs(dl), np.abs(dr))
+    for c in (3.0, -3.0):
+        theta = np.zeros(L.NPARAM)
+        theta[L.NPARAM - 8:] = c
+        out, fb, rep = L.limited_slopes(theta, q5)
+        _, s = L.features(q5)
+        y = vl + c * s
+        mono = dl * dr > 0
+        right = mono & (y * dl > 0)
+        assert right.any() and (mono & ~right).any()
+        assert np.allclose(out[right], np.copysign(np.minimum(np.abs(y), bound), y)[right], rtol=1e-15, atol=0)
+        assert np.array_equal(out[~right], vl[~right])
+        assert np.all(np.abs(out[mono]) <= bound[mono] * (1 + 1e-15))
+        assert np.array_equal(rep, ~(right & (np.abs(y) <= bound)))
+
+
 def test_restriction_conserves_and_keeps_div_b_zero():
     pb = P.turbulence(16, seed=2, mach=1.0, beta=1.0)
     uc = L.restrict_state(pb.uin)
