@@ -59,6 +59,15 @@ __device__ __forceinline__ float van_leer(float dl, float dr)
     return p > 0.0f ? 2.0f * p / (dl + dr) : 0.0f;
 }
 
+// the MC slope (scheme.f90:452-477 with slope_type 2) from the one-sided
+// differences: sign(dl + dr) min(2|dl|, 2|dr|, |dl + dr| / 2), 0 at an extremum
+__device__ __forceinline__ float mc_slope(float dl, float dr)
+{
+    const float c = 0.5f * (dl + dr);
+    const float m = fminf(fminf(2.0f * fabsf(dl), 2.0f * fabsf(dr)), fabsf(c));
+    return dl * dr > 0.0f ? copysignf(m, c) : 0.0f;
+}
+
 // ---------------------------------------------------------------------------
 // bf16 MFMA (gfx950 v_mfma_f32_16x16x32_bf16): D[16x16] += A[16x32] B[32x16].
 // Lane l holds A row (l % 16), k = 8 (l / 16) + 0..7; B column (l % 16),

@@ -133,16 +133,30 @@ def van_leer(dl, dr):
     return np.where(p > 0, 2.0 * p / np.where(p > 0, dl + dr, 1.0), 0.0)
 
 
+def mc_slope(dl, dr):
+    """The MC slope (scheme.f90:452-477 with slope_type 2): sign(dl + dr)
+    min(2|dl|, 2|dr|, |dl + dr| / 2) where dl dr > 0, else 0."""
+    c = 0.5 * (dl + dr)
+    m = np.minimum(np.minimum(2.0 * np.abs(dl), 2.0 * np.abs(dr)), np.abs(c))
+    return np.where(dl * dr > 0, np.copysign(m, c), 0.0)
+
+
+def monotone(q5):
+    """Cells (n,) whose 8 variables have no extremum along the stencil
+    (dl dr >= 0 for every variable): the limiter rule gives them MC."""
+    return ((q5[:, 2] - q5[:, 1]) * (q5[:, 3] - q5[:, 2]) >= 0).all(axis=1)
+
+
 def limited_slopes(theta, q5):
-    """The device rule: the van Leer slope plus the network's correction
-    (its output times the stencil scale), kept in the TVD region: a
-    variable at an extremum or whose slope has the wrong sign takes the van
-    Leer slope, one larger than twice the smaller one-sided difference is
-    clamped to that bound; a cell whose density or pressure would turn
-    non-positive at a face takes van Leer for all 8 variables.  A network
-    that outputs zero is the van Leer limiter.  Returns (slopes (n, 8),
-    fell_back (n,), replaced (n, 8): the variables whose network slope left
-    the TVD region)."""
+    """The device rule: a cell monotone in all 8 variables along the
+    stencil takes the MC slopes.  Elsewhere the van Leer slope plus the
+    network's correction (its output times the stencil scale), kept in the
+    TVD region: a variable at an extremum or whose slope has the wrong sign
+    takes the van Leer slope, one larger than twice the smaller one-sided
+    difference is clamped to that bound.  A cell whose density or pressure
+    would turn non-positive at a face takes van Leer for all 8 variables.
+    Returns (slopes (n, 8), fell_back (n,), replaced (n, 8): the variables
+    of non-monotone cells whose network slope left the TVD region)."""
     X, s = features(q5)
     dl = q5[:, 2] - q5[:, 1]
     dr = q5[:, 3] - q5[:, 2]
@@ -150,8 +164,9 @@ def limited_slopes(theta, q5):
     y = vl + forward(theta, X) * s
     sgn = (dl * dr > 0) & (y * dl > 0)
     bound = 2.0 * np.minimum(np.abs(dl), np.abs(dr))
-    tvd = sgn & (np.abs(y) <= bound)
-    out = np.where(sgn, np.copysign(np.minimum(np.abs(y), bound), y), vl)
+    mono = monotone(q5)[:, None]
+    tvd = mono | (sgn & (np.abs(y) <= bound))
+    out = np.where(mono, mc_slope(dl, dr), np.where(sgn, np.copysign(np.minimum(np.abs(y), bound), y), vl))
     q0 = q5[:, 2]
     pos = np.ones(q5.shape[0], dtype=bool)
     for v in (0, 4):
@@ -258,12 +273,14 @@ def tvd_project(t, dl, dr, margin: float = 1.0):
     return np.where(dl * dr > 0, sg * mag, 0.0)
 
 
-def training_pairs(u_fine: np.ndarray, gamma: float, tvd: bool = False, margin: float = 1.0):
+def training_pairs(u_fine: np.ndarray, gamma: float, tvd: bool = False, margin: float = 1.0,
+                   extrema_only: bool = False):
     """Features X (n, 40) and normalised targets T (n, 8) of every coarse
     cell and direction of one fine state: the difference between the true
     slope and the van Leer slope over the stencil scale, clipped to [-4, 4];
     with tvd, the true slope first projected into the region the limiter
-    rule accepts (tvd_project, with margin)."""
+    rule accepts (tvd_project, with margin); with extrema_only, only the
+    cells the network acts on (not monotone in all 8 variables)."""
     uc = restrict_state(u_fine)
     tgt = target_slopes(u_fine, gamma)
     Xs, Ts = [], []
@@ -275,6 +292,10 @@ def training_pairs(u_fine: np.ndarray, gamma: float, tvd: bool = False, margin: 
         if tvd:
             t = tvd_project(t, dl, dr, margin)
         t = t - van_leer(dl, dr)
-        Ts.append(np.clip(np.where(s > 0, t / np.where(s > 0, s, 1.0), 0.0), -4.0, 4.0))
+        T = np.clip(np.where(s > 0, t / np.where(s > 0, s, 1.0), 0.0), -4.0, 4.0)
+        if extrema_only:
+            keep = ~monotone(q5)
+            X, T = X[keep], T[keep]
+        Ts.append(T)
         Xs.append(X)
     return np.concatenate(Xs).astype(np.float32), np.concatenate(Ts).astype(np.float32)

@@ -72,7 +72,7 @@ def run_ensemble(boxes, nsnap: int = 5, every: int = 6):
 
 def train(out: str = WEIGHTS, steps: int = 6000, batch: int = 4096, seed: int = 0, tvd: bool = True,
           margin: float = 1.0, nboxes: int = 24, sizes=(32, 64, 96, 128), per_snapshot: int = 60000,
-          log=print):
+          extrema_only: bool = False, log=print):
     """per_snapshot > 0 keeps that many samples of each snapshot, drawn
     at random (bounds the memory of large ensembles)."""
     t0 = time.perf_counter()
@@ -82,7 +82,7 @@ def train(out: str = WEIGHTS, steps: int = 6000, batch: int = 4096, seed: int = 
     gs = np.random.Generator(np.random.PCG64(seed + 99))
 
     def pairs(u, gam):
-        X, T = L.training_pairs(u, gam, tvd=tvd, margin=margin)
+        X, T = L.training_pairs(u, gam, tvd=tvd, margin=margin, extrema_only=extrema_only)
         if per_snapshot and X.shape[0] > per_snapshot:
             keep = gs.choice(X.shape[0], per_snapshot, replace=False)
             X, T = X[keep], T[keep]
@@ -123,6 +123,7 @@ def train(out: str = WEIGHTS, steps: int = 6000, batch: int = 4096, seed: int = 
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     np.savez(out, theta=theta, losses=np.concatenate(losses), val_loss=np.float64(vl),
              zero_loss=np.float64(base), samples=np.int64(X.shape[0]), tvd_targets=np.bool_(tvd),
+             extrema_only=np.bool_(extrema_only),
              tvd_margin=np.float64(margin))
     log(f"[train] wrote {out}: validation loss {vl:.4e} (zero-network loss {base:.4e})")
     m.close()
@@ -209,10 +210,13 @@ def main(argv=None):
     ap.add_argument("--boxes", type=int, default=24)
     ap.add_argument("--sizes", default="32,64,96,128", help="fine box sizes to draw from")
     ap.add_argument("--per-snapshot", type=int, default=60000, help="samples kept per snapshot (0: all)")
+    ap.add_argument("--extrema-only", action="store_true",
+                    help="train only on the cells the network acts on (not monotone in all 8 variables)")
     a = ap.parse_args(argv)
     if a.cmd == "train":
         train(a.out, a.steps, tvd=a.tvd_targets, margin=a.tvd_margin, nboxes=a.boxes,
-              sizes=tuple(int(x) for x in a.sizes.split(",")), per_snapshot=a.per_snapshot)
+              sizes=tuple(int(x) for x in a.sizes.split(",")), per_snapshot=a.per_snapshot,
+              extrema_only=a.extrema_only)
     else:
         theta = np.load(a.weights)["theta"]
         res = evaluate(theta, a.n, tend=a.tend, fine_precision=a.fine_precision, kmax=a.kmax)

@@ -216,12 +216,12 @@ int  dmx_batch_last_ms(dmx_batch* batch, double* ms);
  * No counterpart in the reference.  A 40-64-64-8 GELU network maps the
  * scale-free features of the 5-cell stencil of the 8 primitive variables
  * along one direction to corrections of the 8 van Leer slopes of the centre
- * cell (slope = van Leer + output x stencil scale, so a network that outputs
- * zero is the van Leer limiter); it runs on bf16 MFMA (float accumulation)
- * before the trace.  A slope above the TVD bound (twice the smaller
- * one-sided difference) is clamped to it; one of the wrong sign or at an
- * extremum, or a cell whose density or pressure would turn non-positive at
- * a face, falls back to van Leer.  Parameters are 7304 floats: W1[40][64], b1,
+ * cell (slope = van Leer + output x stencil scale); it runs on bf16 MFMA
+ * (float accumulation) before the trace.  A cell monotone in all 8 variables along the direction
+ * takes the MC slopes instead.  A slope above the TVD bound (twice the
+ * smaller one-sided difference) is clamped to it; one of the wrong sign or
+ * at an extremum, or a cell whose density or pressure would turn
+ * non-positive at a face, falls back to van Leer.  Parameters are 7304 floats: W1[40][64], b1,
  * W2[64][64], b2, W3[64][8], b3 (row = input).  dumses-hybrid_amd/dmx/learned.py
  * is the float64 reference of features, network and fallback. */
 int  dmx_set_limiter(dmx_ctx* ctx, const float* theta, int32_t n);

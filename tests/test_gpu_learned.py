@@ -101,9 +101,10 @@ def test_limiter_kernel_matches_reference():
     assert abs(frep - frep_ref) < 0.05
 
 
-def test_zero_network_is_van_leer():
-    """A network that outputs zero leaves the van Leer slope of the float
-    primitives, which the rule always accepts."""
+def test_zero_network_is_mc_or_van_leer():
+    """A network that outputs zero leaves the MC slope of the float
+    primitives in cells monotone in all 8 variables, the van Leer slope
+    elsewhere."""
     pb = P.turbulence(16, seed=13, mach=1.0, beta=1.0)
     p = pb.params.replace(slope_type=4, precision=1)
     print("[gpu] zero network", flush=True)
@@ -114,10 +115,12 @@ def test_zero_network_is_van_leer():
     d.close()
     for dd in range(3):
         q5 = L.stencils(u, p.gamma, dd).astype(np.float32).astype(np.float64)
-        vl = L.van_leer(q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2]).reshape(p.nz, p.ny, p.nx, 8).transpose(3, 0, 1, 2)
+        dl, dr = q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2]
+        want = np.where(L.monotone(q5)[:, None], L.mc_slope(dl, dr), L.van_leer(dl, dr))
+        want = want.reshape(p.nz, p.ny, p.nx, 8).transpose(3, 0, 1, 2)
         _, s = L.features(q5)
         scale = s.reshape(p.nz, p.ny, p.nx, 8).transpose(3, 0, 1, 2)
-        assert np.all(np.abs(got[dd] - vl) <= 1e-5 * scale + 1e-30)
+        assert np.all(np.abs(got[dd] - want) <= 1e-5 * scale + 1e-30)
     assert fb == 0.0
 
 

@@ -45,21 +45,43 @@ def test_features_are_scale_free():
     assert (X3 == 0).all() and (s3 == 0).all()
 
 
-def test_zero_network_falls_back_to_van_leer():
+def test_mc_slope_is_the_reference_limiter():
+    """mc_slope against the reference's slope formula with slope_type 2
+    (scheme.f90:452-477: dlft = 2 dl, drgt = 2 dr, dcen = (dlft + drgt) / 4,
+    dq = sign(dcen) min(min(|dlft|, |drgt|) [dlft drgt > 0], |dcen|)),
+    brute force on random differences with ties and zeros."""
+    g = np.random.Generator(np.random.PCG64(7))
+    dl = np.round(g.normal(size=20000), 1)
+    dr = np.round(g.normal(size=20000), 1)
+    dlft, drgt = 2.0 * dl, 2.0 * dr
+    dcen = 0.5 * (dlft + drgt) / 2.0
+    dlim = np.where(dlft * drgt <= 0, 0.0, np.minimum(np.abs(dlft), np.abs(drgt)))
+    ref = np.copysign(1.0, dcen) * np.minimum(dlim, np.abs(dcen))
+    assert np.array_equal(L.mc_slope(dl, dr), ref)
+
+
+def test_zero_network_is_mc_or_van_leer():
+    """A network that outputs zero: MC in cells monotone in all 8 variables,
+    van Leer elsewhere."""
     g = np.random.Generator(np.random.PCG64(6))
-    q5 = np.abs(g.standard_normal((200, 5, 8))) + 0.5
+    q5 = np.abs(g.standard_normal((4000, 5, 8))) + 0.5
+    q5[:100] = np.sort(q5[:100], axis=1)          # some monotone cells
     out, fb, rep = L.limited_slopes(np.zeros(L.NPARAM), q5)
-    vl = L.van_leer(q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2])
-    assert np.array_equal(out, vl)
+    dl, dr = q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2]
+    mono = L.monotone(q5)
+    assert mono[:100].all() and (~mono).sum() > 3000
+    assert np.array_equal(out, np.where(mono[:, None], L.mc_slope(dl, dr), L.van_leer(dl, dr)))
 
 
 def test_rule_clamps_to_the_tvd_bound_and_falls_back():
-    """A network whose output is a constant c (only the last bias set):
-    where van Leer + c s has the sign of the one-sided differences it is
-    clamped to twice the smaller one; where it has the wrong sign, and at
-    extrema, the slope is van Leer."""
+    """A network whose output is a constant c (only the last bias set), in
+    cells with an extremum in some variable: where van Leer + c s has the
+    sign of the one-sided differences it is clamped to twice the smaller
+    one; where it has the wrong sign, and at extrema, the slope is van
+    Leer.  Monotone cells take MC whatever the network says."""
     g = np.random.Generator(np.random.PCG64(8))
     q5 = np.abs(g.standard_normal((4000, 5, 8))) + 0.5
+    q5[:100] = np.sort(q5[:100], axis=1)          # some monotone cells
     dl, dr = q5[:, 2] - q5[:, 1], q5[:, 3] - q5[:, 2]
     vl = L.van_leer(dl, dr)
     bound = 2.0 * np.minimum(np.abs(dl), np.abs(dr))
@@ -69,13 +91,16 @@ def test_rule_clamps_to_the_tvd_bound_and_falls_back():
         out, fb, rep = L.limited_slopes(theta, q5)
         _, s = L.features(q5)
         y = vl + c * s
+        cellmono = np.broadcast_to(L.monotone(q5)[:, None], y.shape)
         mono = dl * dr > 0
-        right = mono & (y * dl > 0)
-        assert right.any() and (mono & ~right).any()
+        right = mono & (y * dl > 0) & ~cellmono
+        wrong = ~right & ~cellmono
+        assert right.any() and (mono & wrong).any() and cellmono.any()
         assert np.allclose(out[right], np.copysign(np.minimum(np.abs(y), bound), y)[right], rtol=1e-15, atol=0)
-        assert np.array_equal(out[~right], vl[~right])
+        assert np.array_equal(out[wrong], vl[wrong])
+        assert np.array_equal(out[cellmono], L.mc_slope(dl, dr)[cellmono])
         assert np.all(np.abs(out[mono]) <= bound[mono] * (1 + 1e-15))
-        assert np.array_equal(rep, ~(right & (np.abs(y) <= bound)))
+        assert np.array_equal(rep, ~cellmono & ~(right & (np.abs(y) <= bound)))
 
 
 def test_restriction_conserves_and_keeps_div_b_zero():
