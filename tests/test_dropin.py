@@ -63,8 +63,8 @@ def test_dropin_reports_device_errors_on_the_host(tmp_path):
     """Without a usable device the shim stops with the library's message
     (the reference has no error convention; the shim turns a nonzero status
     into error stop)."""
-    if os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"):
-        pytest.skip("a GPU is visible here")
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU driver is present here: the driver would run, not stop")
     pb = P.random_smooth(8, 8, 8, seed=1)
     p = pb.params
     O.write_input(str(tmp_path / "input"), p)
