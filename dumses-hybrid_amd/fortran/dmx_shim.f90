@@ -114,6 +114,14 @@ contains
     character(len=16) :: prec
     integer :: d, plen, pstat
     if (c_associated(ctx)) return
+    ! source terms (rhs, src/dumses.f90:84-93) run on the host copy between
+    ! godunov and the ghost fill, and the gravity predictor inside godunov
+    ! (src/subroutines/scheme.f90:141) has no device counterpart: refuse
+    ! rather than step with a stale host state
+    if (rhs) then
+       print '(A)', 'dmx: rhs=.true. (source_term) is not supported by the device step'
+       error stop 1
+    endif
     p%nx = nx; p%ny = ny; p%nz = nz
 #if ISO == 1
     p%iso = 1

@@ -210,6 +210,53 @@ def run_reference(p: Params, uin: np.ndarray, nsteps: int, dumpevery: int = 0,
         shutil.rmtree(work, ignore_errors=True)
 
 
+def parse_history(text: str):
+    """Rows of a history.txt written by the reference's history routine
+    (src/subroutines/history.f90:146-163: time, dt, then mass, maxwell,
+    reynolds, max+rey, magp, meanBx, meanBy, meanBz, divB in E14.5).  Returns
+    (values, half_ulps): float arrays of shape (rows, 11), the second the
+    rounding bound of each printed value, half a unit in its last printed
+    digit."""
+    vals, hulps = [], []
+    for line in text.splitlines():
+        if not line.strip() or line.lstrip().startswith("#"):
+            continue
+        v, h = [], []
+        for tok in line.split():
+            mant, _, exp = tok.upper().partition("E")
+            digits = len(mant.split(".")[1]) if "." in mant else 0
+            v.append(float(tok))
+            h.append(0.5 * 10.0 ** (int(exp or 0) - digits))
+        vals.append(v)
+        hulps.append(h)
+    return np.array(vals), np.array(hulps)
+
+
+def run_reference_history(p: Params, uin: np.ndarray, time: float = 0.0):
+    """The reference's own history routine on the state uin (injected through
+    its restart reader): returns the printed row (values, half_ulps) as
+    parse_history, columns time, dt, mass .. divB."""
+    exe = os.path.join(REF_DIR, f"iso{int(p.iso)}", "ref_harness")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"compiled reference missing: {exe} (make -f oracle/ref.mk)")
+    work = tempfile.mkdtemp(prefix="dmxrefh_")
+    try:
+        write_input(os.path.join(work, "input"), p)
+        with open(os.path.join(work, "harness"), "w") as f:
+            f.write("&harness_params\n nsteps=0\n dumpevery=0\n mode='history'\n/\n")
+        d = Dump(p.nx, p.ny, p.nz, time, 0.0, p.dx, p.dy, p.dz,
+                 (p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax), p.gamma, p.ciso,
+                 np.ascontiguousarray(uin, dtype=np.float64))
+        write_dump(dump_path(1, 0, work), d)
+        r = subprocess.run([exe], cwd=work, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference history failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+        vals, hulps = parse_history(open(os.path.join(work, "history.txt")).read())
+        return vals[-1], hulps[-1]
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def run_reference_kernels(iso: int, gamma: float, ciso: float, dt: float,
                           s1: np.ndarray, s2: np.ndarray) -> np.ndarray:
     """Feeds n 1D state pairs s1 (n,16) and n 2D corner sets s2 (n,32) through

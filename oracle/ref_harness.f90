@@ -15,6 +15,9 @@
 ! A second mode ("kernels") feeds random states through the reference's 1D and
 ! 2D Riemann solvers (src/modules/solvers/solver.f90, solver_magnetic.f90) and
 ! writes their outputs, to pin the solver restatements function by function.
+! A third mode ("history") injects the state and calls the reference's own
+! history routine (src/subroutines/history.f90:18-165) once, which writes its
+! volume sums to history.txt at the precision of its format.
 !===============================================================================
 program ref_harness
   use params
@@ -44,6 +47,12 @@ program ref_harness
   nspec = 0
   call init_param
   call init
+
+  if (trim(mode) == 'history') then
+     dt = 0.d0
+     call history
+     stop
+  endif
 
   open(unit=21, file='dt.raw', access='stream', form='unformatted', &
        status='replace')

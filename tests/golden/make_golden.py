@@ -81,19 +81,55 @@ def write_step(name, p, uin, steps):
 
 def write_loop():
     """Field-loop advection, 128x128x4, 200 steps: per-step dt, the history
-    sums at the end, a 1-in-4 subsample of the final interior and the sha256
-    of the full final interior (the full state is 4 MB)."""
+    sums of the final state as the reference's own history routine prints
+    them (mass .. divB, with the half-unit rounding bound of each printed
+    value), a 1-in-4 subsample of the final interior and the sha256 of the
+    full final interior (the full state is 4 MB)."""
     pb = P.magnetic_loop()
     p = pb.params
     ref, dts, _ = O.run_reference(p, pb.uin, 200)
-    o = O.Oracle(p, ref)
-    hist = o.history()
+    hv, hh = O.run_reference_history(p, ref, float(np.sum(dts)))
     inner = np.ascontiguousarray(P.interior(ref))
     np.savez_compressed(
         os.path.join(HERE, "magnetic_loop_200.npz"), params=json.dumps(p.to_dict()),
-        dts=dts, history=hist, sub=inner[:, :, ::4, ::4],
+        dts=dts, history=hv[2:], history_hulp=hh[2:], sub=inner[:, :, ::4, ::4],
         sha256=hashlib.sha256(inner.tobytes()).hexdigest())
     print(f"magnetic_loop_200: dt {dts[0]:.6g} .. {dts[-1]:.6g}")
+
+
+def history_cases():
+    """States on which the reference's history routine is run: a random
+    smooth field (distinct By/Bz and v/w everywhere, so a swapped component
+    changes every stress), the Orszag-Tang 16^3 state after 10 steps and the
+    shearing box with FARGO after 8 steps (both from their step fixtures)."""
+    rs = P.random_smooth(16, 12, 8, seed=11)
+    yield "random_smooth", rs.params, rs.uin, 0.25
+    for name, step in (("orszag_tang_16", 10), ("shear_fargo", 8)):
+        z = np.load(os.path.join(HERE, name + ".npz"), allow_pickle=False)
+        d = json.loads(str(z["params"]))
+        d["bdtype"] = tuple(d["bdtype"])
+        p = P.Params(**d)
+        u = np.zeros((11, p.nz + 6, p.ny + 6, p.nx + 6))
+        u[:8, 3:-3, 3:-3, 3:-3] = z[f"out_{step:04d}"]
+        P.fill_periodic(u)
+        yield name, p, u, float(np.sum(z["dts"][:step]))
+
+
+def write_history():
+    """history_reference.npz: per case the interior state, its time and the
+    row the reference's own history routine (src/subroutines/history.f90,
+    called by the harness on the injected state) prints for it."""
+    out = {}
+    names = []
+    for name, p, u, t in history_cases():
+        hv, hh = O.run_reference_history(p, u, t)
+        out[f"{name}_params"] = json.dumps(p.to_dict())
+        out[f"{name}_ic"] = P.interior(u)
+        out[f"{name}_row"] = hv
+        out[f"{name}_hulp"] = hh
+        names.append(name)
+        print(f"history {name}: maxwell {hv[3]:.6g} reynolds {hv[4]:.6g}")
+    np.savez_compressed(os.path.join(HERE, "history_reference.npz"), cases=np.array(names), **out)
 
 
 def random_states(n, seed):
@@ -184,7 +220,8 @@ def copy_mri_history():
 
 
 def main(only=None):
-    """Writes every fixture, or only the step fixtures named in `only`."""
+    """Writes every fixture, or only those named in `only` (step fixture
+    names, "loop", "history")."""
     if not (O.reference_available(0) and O.reference_available(1)):
         raise SystemExit("build the reference first: make -f oracle/ref.mk")
     if not only:
@@ -193,8 +230,10 @@ def main(only=None):
     for name, p, u, steps in step_cases():
         if not only or name in only:
             write_step(name, p, u, steps)
-    if not only:
+    if not only or "loop" in only:
         write_loop()
+    if not only or "history" in only:
+        write_history()
 
 
 if __name__ == "__main__":

@@ -15,7 +15,7 @@ import pytest
 import oracle as O
 from dmx import problems as P
 from dmx.dumpfile import Dump, dump_path, read_dump, write_dump
-from goldens import load_step
+from goldens import assert_history_matches_printed, history_scales, load_step
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(REPO, "dumses-hybrid_amd", "fortran", "_build")
@@ -76,6 +76,25 @@ def test_dropin_reports_device_errors_on_the_host(tmp_path):
     assert "dmx:" in r.stdout + r.stderr
 
 
+@pytest.mark.skipif(not os.path.exists(exe(0)), reason="drop-in driver not built")
+def test_dropin_refuses_source_terms(tmp_path):
+    """rhs=.true. (source_term between godunov and the ghost fill,
+    src/dumses.f90:84-93) has no device counterpart: the shim stops with a
+    message before it creates a context, instead of stepping on a stale
+    host copy."""
+    pb = P.random_smooth(8, 8, 8, seed=1)
+    p = pb.params
+    O.write_input(str(tmp_path / "input"), p)
+    txt = (tmp_path / "input").read_text().replace(" eta=", " rhs=.true.\n eta=")
+    (tmp_path / "input").write_text(txt)
+    write_dump(dump_path(1, 0, str(tmp_path)), Dump(p.nx, p.ny, p.nz, 0.0, 0.0, p.dx, p.dy, p.dz,
+                                                    (p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax),
+                                                    p.gamma, p.ciso, pb.uin))
+    r = subprocess.run([exe(0)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "rhs=.true. (source_term) is not supported" in r.stdout + r.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["orszag_tang_16", "step_hlld_zerogradient", "shear_fargo",
                                   "shear_nofargo", "shear_fargo_iso", "diss_both",
@@ -90,4 +109,14 @@ def test_dropin_driver_matches_reference_bitwise(name):
     assert abs(d.time - np.cumsum(dts)[n - 1]) == 0.0
     got = P.interior(d.uin)
     assert np.array_equal(got, outs[n]), float(np.abs(got - outs[n]).max())
-    assert len(hist.strip().splitlines()) >= 2   # header + the synchronised row
+    # the row the reference's own history routine (history.f90, linked into
+    # the drop-in driver) wrote for that same state, against dmx_history on
+    # the device and dmxo_history, at the printed precision
+    rows, hulps = O.parse_history(hist)
+    assert rows.shape[0] == 1 and rows[0, 0] == pytest.approx(d.time, rel=1e-5)
+    from dmx import abi
+    dom = abi.Domain(p, d.uin)
+    scale = history_scales(p, d.uin)
+    assert_history_matches_printed(dom.history(), rows[0], hulps[0], scale, name + " (device)")
+    assert_history_matches_printed(O.Oracle(p, d.uin).history(), rows[0], hulps[0], scale, name + " (oracle)")
+    dom.close()

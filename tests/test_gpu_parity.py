@@ -13,7 +13,8 @@ import pytest
 import oracle as O
 from dmx import abi
 from dmx import problems as P
-from goldens import STEP_FIXTURES, load_step
+from goldens import (STEP_FIXTURES, assert_history_matches_printed, history_scales, load_history_cases,
+                     load_step)
 
 pytestmark = pytest.mark.gpu
 
@@ -112,6 +113,17 @@ def test_tuning_leaves_results_unchanged():
     b = abi.Domain(p, u)
     tb, _ = b.advance(5, 0.0)
     assert ta == tb and np.array_equal(a.download(), b.download())
+
+
+@pytest.mark.parametrize("case", [c[0] for c in load_history_cases()])
+def test_history_sums_match_reference_printed(case):
+    """dmx_history on the device against the row the reference's own history
+    routine printed for the same state (history_reference.npz), at the
+    printed precision."""
+    name, p, u, t, row, hulp = [c for c in load_history_cases() if c[0] == case][0]
+    print(f"[gpu] history {name}", flush=True)
+    d = abi.Domain(p, u)
+    assert_history_matches_printed(d.history(), row, hulp, history_scales(p, u), name)
 
 
 def test_history_sums_match_oracle():
