@@ -32,21 +32,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# algorithmic bytes per interior cell of each kernel group: its inputs read
-# once and its outputs written once (8-byte words in float64; T-size words
-# for the bulk in float32, face B always float64).  The trace summary has 47
-# words per cell, the face fluxes 18, the edge EMFs 3.
-def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
-    hydro, bfield = 5 * wt, 3 * wb
+# Algorithmic bytes per zone-cycle: one read and one write of the stored
+# words of every interior cell (ghosts, dt and re-reads excluded): 8 float64
+# words in float64 mode; in float32-bulk mode rho, rho*u, rho*v, rho*w in
+# float32 and E, Bx, By, Bz in float64 (4 x 4 B + 4 x 8 B).
+ALG_BYTES_PER_ZONE_CYCLE = {"f64": 128, "f32": 96}
+
+
+# bytes per interior cell of each kernel group's own staging, its inputs
+# read once and its outputs written once (8-byte words in float64; T-size
+# words for the bulk in float32, face B and E always float64).  The trace
+# summary has 47 words per cell, the face fluxes 18, the edge EMFs 3.  This
+# is this design's intermediate traffic, not algorithmic work: reported as
+# stage_bytes_per_cell beside the roofline.
+def stage_bytes(group: str, wt: int, wb: int = 8) -> float:
+    state = 4 * wt + 4 * wb
     table = {
-        "compute_dt": hydro + bfield,
-        "trace": hydro + bfield + 47 * wt,
+        "compute_dt": state,
+        "trace": state + 47 * wt,
         "faces": 47 * wt + 18 * wt,
         "edges": 47 * wt + 3 * wt,
         "flux": 47 * wt + 18 * wt + 3 * wt,           # faces and edges in one pass
-        "update_ct": 2 * (hydro + bfield) + 18 * wt + 3 * wt,
+        "fused": state + 18 * wt + 3 * wt,            # trace + faces + edges in one pass
+        "update_ct": 2 * state + 18 * wt + 3 * wt,
         # learned limiter: state -> 8 float primitives -> 24 float slopes
-        "learned_limiter": hydro + bfield + 8 * 4 + 8 * 4 + 24 * 4,
+        "learned_limiter": state + 8 * 4 + 8 * 4 + 24 * 4,
         "boundary": 0.0,
     }
     return float(table.get(group, 0.0))
@@ -56,12 +66,10 @@ def algo_bytes(group: str, wt: int, wb: int = 8) -> float:
 # x 1 KiB from the two rocprofv3 counter runs at 256^3 summarised in
 # profiles/mhd_step_ot256_hbm_bytes.md (FETCH_SIZE scaled by 2.00, WRITE_SIZE
 # by 1.00 against the known-size calibration kernels of the same runs), for
-# the launch shapes the autotuner keeps at 256^3 (float64: fused flux kernel,
-# 8 planes; float32: carrying flux kernel, 8 planes; z-marching trace on a
-# 32x8 column tile, 16 planes in float64 and 8 in float32).  The float64
-# update is k_update_dt (the next dt fused in on periodic domains), whose
-# compute_dt group is then k_dt_final alone after the first step; the
-# compute_dt figures are those of k_dt_min.
+# the launch shapes the autotuner keeps at 256^3.  Constants copied from
+# that file; COUNTERS_SOURCE names the file and the commit whose kernels it
+# measured.
+COUNTERS_SOURCE = "profiles/mhd_step_ot256_hbm_bytes.md, profiles/mhd_step_ot256_sq_counters.md (kernels of commit 8a235bc)"
 MEASURED_TRAFFIC = {
     "f64": {"trace": 571.0, "flux": 1653.1, "update_ct": 416.8, "compute_dt": 86.0,
             "boundary": 17.7},
@@ -81,18 +89,23 @@ MEASURED_VALU_BUSY = {
 
 
 def roofline(prof: dict, cells: int, prec: str) -> dict | None:
-    """HBM roofline of the dominant kernel group of a profiled run."""
+    """HBM roofline of the dominant kernel group of a profiled run: the
+    algorithmic bytes of the zone-cycles one launch advances (cells x
+    ALG_BYTES_PER_ZONE_CYCLE) over that launch's mean duration (HIP events on
+    the library's stream), against the 8 TB/s HBM peak."""
     if not prof:
         return None
     dom = max(prof, key=prof.get)
     ms = prof[dom]
-    bpc = algo_bytes(dom, 8 if prec == "f64" else 4)
-    ach = bpc * cells / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    b = ALG_BYTES_PER_ZONE_CYCLE[prec]
+    ach = b * cells / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     meas = MEASURED_TRAFFIC.get(prec, {}).get(dom)
     traffic = meas * cells if meas is not None else None
     return dict(bound="hbm", kernel=dom, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=ach / HBM_PEAK_GBS, traffic=traffic, algo_bytes_per_cell=bpc,
+                frac=ach / HBM_PEAK_GBS, traffic=traffic, algo_bytes_per_zone_cycle=b, cells=cells,
+                stage_bytes_per_cell=stage_bytes(dom, 8 if prec == "f64" else 4),
                 valu_busy=MEASURED_VALU_BUSY.get(prec, {}).get(dom),
+                counters_source=COUNTERS_SOURCE if meas is not None else None,
                 traffic_gbs=(traffic / (ms * 1e-3) / 1e9 if traffic and ms > 0 else None),
                 kernel_ms=ms, step_kernel_ms=prof)
 
@@ -104,10 +117,15 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
 TUNED_256 = {0: ((0, 8), (1, 16)), 1: ((2, 8), (1, 8))}
 
 
-def run_gpu(pb, steps, warmup, precision, limiter=None, tune=True):
-    """Warm-up, then the timed region: `steps` steps of the instrumented
-    advance (HIP events around every kernel group on the library's stream,
-    read once after the run), bracketed by device synchronisation."""
+REPEATS = 5
+
+
+def run_gpu(pb, steps, warmup, precision, limiter=None, tune=True, repeats=REPEATS):
+    """Warm-up, then `repeats` timed regions of `steps` steps each of the
+    instrumented advance (HIP events around every kernel group on the
+    library's stream, read once after the run), each bracketed by device
+    synchronisation; the median repeat is reported, per-kernel times are
+    the medians over the repeats."""
     from dmx import abi
     p = pb.params.replace(precision=precision)
     if limiter is not None:
@@ -124,14 +142,21 @@ def run_gpu(pb, steps, warmup, precision, limiter=None, tune=True):
         tuned_ms = None
         log(f"[bench] fixed shapes: face/edge {d.shape}, trace {d.trace_shape}")
     t, _ = d.advance(warmup, 0.0)          # ends with a stream synchronisation
-    t0 = time.perf_counter()
-    prof = d.profile(steps, t)             # ends with a stream synchronisation
-    wall = time.perf_counter() - t0
-    dev_ms = d.last_advance_ms()
+    walls, devs, profs = [], [], []
+    for r in range(repeats):
+        t0 = time.perf_counter()
+        prof = d.profile(steps, t)         # ends with a stream synchronisation
+        walls.append(time.perf_counter() - t0)
+        devs.append(d.last_advance_ms())
+        profs.append(prof)
+        t = d.time
+        log(f"[bench]   repeat {r + 1}/{repeats}: {1e3 * walls[-1] / steps:.3f} ms/step")
+    med = sorted(range(repeats), key=lambda q: walls[q])[repeats // 2]
+    prof = {k: float(np.median([pr[k] for pr in profs])) for k in profs[0]}
     state = d.download()
     d.close()
-    return dict(wall=wall, dev_ms=dev_ms, prof=prof, state=state, tuned_ms=tuned_ms, shape=d.shape,
-                trace_shape=d.trace_shape)
+    return dict(wall=walls[med], walls=walls, dev_ms=devs[med], prof=prof, state=state, tuned_ms=tuned_ms,
+                shape=d.shape, trace_shape=d.trace_shape, steps_total=warmup + repeats * steps)
 
 
 def run_ensemble(steps=50, warmup=2, nboxes=48):
@@ -209,9 +234,13 @@ def cpu_baseline(n=128, steps=3, threads=(1, 2, 4, 8, 16)):
             shutil.rmtree(work, ignore_errors=True)
         best = max(sweep, key=sweep.get)
         return dict(value=sweep[best], unit="zone-cycles/s", cores=best, kind="reference",
-                    sample=f"Orszag-Tang {n}^3 (same initial field at 1/8 of the cells), {steps} steps "
-                           f"(compute_dt+godunov+boundary), step loop only, reference OpenMP -O3 build; "
-                           f"best of OMP_NUM_THREADS in {sorted(sweep)}",
+                    sample=f"Orszag-Tang {n}^3 (the same initial field as the 256^3 workload at 1/8 of "
+                           f"its cells), {steps} steps (compute_dt+godunov+boundary), step loop only, "
+                           f"reference OpenMP -O3 build; best of OMP_NUM_THREADS in {sorted(sweep)}",
+                    sample_cells=n ** 3, sample_steps=steps, host_cpus=ncpu,
+                    sweep_note=f"the sweep stops at {max(sweep)} threads: this host shows {ncpu} CPUs, "
+                               f"a GPU job's share is 16, and the reference's throughput falls with "
+                               f"threads beyond 1-2 (OpenMP regions entered inside every routine)",
                     threads_sweep={str(k): v for k, v in sweep.items()})
     o = O.Oracle(pb.params, pb.uin)
     t0 = time.perf_counter()
@@ -226,6 +255,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--repeats", type=int, default=REPEATS,
+                    help="timed regions of --steps steps each; the median is reported")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--workload", choices=("ot", "shear"), default="ot",
                     help="ot: 3D Orszag-Tang n^3 (the metric's workload); shear: the shearing-box "
@@ -266,13 +297,14 @@ def main():
         workload = f"3D Orszag-Tang {n}^3 periodic, HLLD/HLLD, MC slopes, CFL 0.7"
     cells = pb.params.nx * pb.params.ny * pb.params.nz
     log("[bench] float64 run")
-    r64 = run_gpu(pb, a.steps, a.warmup, 0, tune=not a.no_tune)
+    r64 = run_gpu(pb, a.steps, a.warmup, 0, tune=not a.no_tune, repeats=a.repeats)
     v64 = cells * a.steps / r64["wall"]
-    log(f"[bench] float64: {v64:.4g} zone-cycles/s, {1e3 * r64['wall'] / a.steps:.3f} ms/step")
+    log(f"[bench] float64: {v64:.4g} zone-cycles/s, {1e3 * r64['wall'] / a.steps:.3f} ms/step "
+        f"(median of {a.repeats})")
     out = {}
     if not a.no_fp32:
         log("[bench] float32-bulk run")
-        r32 = run_gpu(pb, a.steps, a.warmup, 1, tune=not a.no_tune)
+        r32 = run_gpu(pb, a.steps, a.warmup, 1, tune=not a.no_tune, repeats=a.repeats)
         v32 = cells * a.steps / r32["wall"]
         u64, u32 = P.interior(r64["state"]), P.interior(r32["state"])
         names = ["rho", "rhou", "rhov", "rhow", "E", "Bx", "By", "Bz"]
@@ -283,15 +315,16 @@ def main():
         norms = [np.abs(u64[0]).sum(), mom, mom, mom, np.abs(u64[4]).sum(), mag, mag, mag]
         l1 = {names[v]: float(np.abs(u32[v] - u64[v]).sum() / norms[v]) for v in range(8)}
         out["fp32"] = dict(value=v32, ms_per_step=1e3 * r32["wall"] / a.steps, speedup_vs_fp64=v32 / v64,
+                           ms_per_step_repeats=[1e3 * w / a.steps for w in r32["walls"]],
                            l1_rel_vs_fp64=l1,
-                           dtype="f32 bulk, f64 face-B/CT/pressure recovery/sums")
+                           dtype="f32 bulk; f64 E, face B/CT, pressure recovery, dt and history sums")
         log(f"[bench] float32: {v32:.4g} zone-cycles/s ({v32 / v64:.2f}x), L1 {l1}")
     if not a.no_fp32 and not a.no_learned:
         from dmx import train_limiter as TL
         if os.path.exists(TL.WEIGHTS):
             log("[bench] float32-bulk run with the learned limiter")
             theta = np.load(TL.WEIGHTS)["theta"]
-            rl = run_gpu(pb, a.steps, a.warmup, 1, limiter=theta, tune=not a.no_tune)
+            rl = run_gpu(pb, a.steps, a.warmup, 1, limiter=theta, tune=not a.no_tune, repeats=1)
             vl = cells * a.steps / rl["wall"]
             ul = P.interior(rl["state"])
             l1l = {names[v]: float(np.abs(ul[v] - u64[v]).sum() / norms[v]) for v in range(8)}
@@ -306,6 +339,7 @@ def main():
     res = {
         "metric": METRIC, "value": v64, "unit": "zone-cycles/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * r64["wall"] / a.steps,
+        "repeats": a.repeats, "ms_per_step_repeats": [1e3 * w / a.steps for w in r64["walls"]],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload, "cells": cells, "parallelism": "single domain, 1 GPU"},
