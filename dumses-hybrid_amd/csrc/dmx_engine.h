@@ -44,10 +44,10 @@ public:
 
 // kernel groups timed by dmx_profile
 enum KernelGroup { KG_DT = 0, KG_TRACE, KG_FACES, KG_EDGES, KG_SHEARFLUX, KG_UPDATE, KG_FARGO, KG_BOUNDARY,
-                   KG_FLUX, KG_DISSIPATION, KG_LEARNED, KG_N };
+                   KG_FLUX, KG_DISSIPATION, KG_LEARNED, KG_FUSED, KG_N };
 static const char* const kKernelGroupNames[KG_N] = {
     "compute_dt", "trace", "faces", "edges", "shearing_flux", "update_ct", "fargo", "boundary", "flux",
-    "dissipation", "learned_limiter"};
+    "dissipation", "learned_limiter", "fused"};
 
 // set by the engines, read by dmx_last_error()
 void set_error(const char* fmt, ...);

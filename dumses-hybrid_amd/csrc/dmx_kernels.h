@@ -4,10 +4,12 @@
 //   index(i,j,k) = (i+2) + px*((j+2) + gy*(k+2)),  i in [-2, nx+3] (the
 //   reference's uin index range), px = nx+6 rounded up to 16 elements so
 //   every x-row starts 128-byte aligned; x is the fastest index (coalesced).
-//   U  : 8 planes -- rho, rho*u, rho*v, rho*w, E in the bulk type T, and the
-//        left-face fields Bx_L, By_L, Bz_L always in float64 (the CT face-B
-//        accumulation stays float64 in both precisions).  The reference's
-//        right faces are implied: B_R(i) = B_L(i+1).
+//   U  : 8 planes -- rho, rho*u, rho*v, rho*w in the bulk type T; the total
+//        energy E and the left-face fields Bx_L, By_L, Bz_L always in float64
+//        (E: the pressure recovery at low beta subtracts the magnetic energy
+//        from it; B: the CT face-B accumulation stays float64 in both
+//        precisions).  The reference's right faces are implied:
+//        B_R(i) = B_L(i+1).
 //   S  : 47 planes of T, the per-cell trace summary written by the trace
 //        kernel for cells [0, n+1]^3 (see TraceSlot).
 //   F  : 18 planes of T, x/y/z face fluxes of rho, rho*u, rho*v, rho*w, E
@@ -57,7 +59,8 @@ struct Phys {
 };
 
 template <typename T> struct Fields {
-    T* uh;          // 5 hydro planes
+    T* uh;          // 4 planes: rho, rho*u, rho*v, rho*w
+    double* ue;     // total energy E
     double* ub;     // 3 face-B planes
     T* s;           // NSLOT planes
     T* f;           // 18 planes

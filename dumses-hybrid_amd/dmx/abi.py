@@ -214,7 +214,7 @@ class Domain:
         _check(lib().dmx_last_advance_ms(self.h, ctypes.byref(ms)), "dmx_last_advance_ms")
         return ms.value
 
-    def profile(self, nsteps: int, time: float | None = None, ngroups: int = 11) -> dict:
+    def profile(self, nsteps: int, time: float | None = None, ngroups: int = 12) -> dict:
         """Mean device ms per step of each kernel group over nsteps."""
         t = ctypes.c_double(self.time if time is None else time)
         ms = np.zeros(ngroups)

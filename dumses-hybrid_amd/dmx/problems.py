@@ -207,6 +207,42 @@ def magnetic_loop(nx: int = 128, ny: int = 128, nz: int = 4, **kw) -> Problem:
     return Problem(p, assemble(p, one, 2.0 * one, one, zero, one, bx, by, bz), "magnetic_loop")
 
 
+def sound_wave(n: int = 64, beta: float = 1e-3, amp: float = 1e-3, **kw) -> Problem:
+    """A small sound wave in a strong uniform field (low beta) on the
+    periodic unit box: rho = 1, P = 1, B uniform along (1, 0.5, 0.25) with
+    |B|^2 = 2 P / beta, and along x the acoustic perturbation of amplitude
+    amp: drho = amp rho sin(2 pi x), dvx = amp c_s sin(2 pi x),
+    dP = amp gamma P sin(2 pi x), c_s = sqrt(gamma P / rho).  With P = 1 the
+    total energy is 1/beta times the internal energy, so the float64
+    pressure recovery E - B^2/2 is what keeps the pressure's digits."""
+    p = Params(nx=n, ny=n, nz=n, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0,
+               zmin=0.0, zmax=1.0, gamma=5.0 / 3.0, courant=0.7).replace(**kw)
+    xc, yc, zc, xf, yf, zf = _mesh(p)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    shape = (p.nz, p.ny, p.nx)
+    cs = np.sqrt(p.gamma)
+    s = np.sin(2.0 * np.pi * X)
+    rho = 1.0 + amp * s
+    vx = amp * cs * s
+    pres = 1.0 + amp * p.gamma * s
+    b0 = np.sqrt(2.0 / beta)
+    d = np.array([1.0, 0.5, 0.25])
+    d = d / np.sqrt((d * d).sum())
+    zero = np.zeros(shape)
+    return Problem(p, assemble(p, rho, vx, zero, zero, pres, zero + b0 * d[0], zero + b0 * d[1],
+                               zero + b0 * d[2]), "sound_wave")
+
+
+def pressure(p: Params, u: np.ndarray) -> np.ndarray:
+    """Interior gas pressure (gamma - 1)(E - rho v^2/2 - B^2/2) of a state,
+    with the face-averaged cell field (scheme.f90:55-65), in float64."""
+    r = u[0, 3:-3, 3:-3, 3:-3]
+    ek = 0.5 * (u[1, 3:-3, 3:-3, 3:-3] ** 2 + u[2, 3:-3, 3:-3, 3:-3] ** 2 + u[3, 3:-3, 3:-3, 3:-3] ** 2) / r
+    b = [0.5 * (u[5 + c, 3:-3, 3:-3, 3:-3] + u[8 + c, 3:-3, 3:-3, 3:-3]) for c in range(3)]
+    em = 0.5 * (b[0] ** 2 + b[1] ** 2 + b[2] ** 2)
+    return (p.gamma - 1.0) * (u[4, 3:-3, 3:-3, 3:-3] - ek - em)
+
+
 def splitmix64(seed: int, n: int) -> np.ndarray:
     """n uniform doubles in [0,1) from the SplitMix64 generator (Steele, Lea &
     Flood 2014): state_i = seed + (i+1)*0x9E3779B97F4A7C15 mod 2**64, output

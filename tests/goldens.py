@@ -81,3 +81,15 @@ def assert_history_matches_printed(got, row, hulp, scale, what=""):
     for q in range(9):
         tol = hulp[2 + q] + 1e-13 * scale[q]
         assert abs(got[q] - row[2 + q]) <= tol, (what, HIST_COLUMNS[q], got[q], row[2 + q], tol)
+
+
+def assert_f32_close(u, us, rtol=5e-6):
+    """float32 states that may differ by the contraction of inlined copies
+    of the same arithmetic: within rtol of the largest magnitude of the
+    variable (rho, E) or of its vector (momentum, B)."""
+    iu, ir = P.interior(u), P.interior(us)
+    mom = np.sqrt((ir[1:4] ** 2).sum(axis=0)).max()
+    mag = np.sqrt((ir[5:8] ** 2).sum(axis=0)).max()
+    scales = [np.abs(ir[0]).max(), mom, mom, mom, np.abs(ir[4]).max(), mag, mag, mag]
+    for v in range(8):
+        assert np.abs(iu[v] - ir[v]).max() <= rtol * scales[v], v

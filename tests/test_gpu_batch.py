@@ -7,6 +7,7 @@ import pytest
 
 from dmx import abi
 from dmx import problems as P
+from goldens import assert_f32_close
 
 pytestmark = pytest.mark.gpu
 
@@ -35,7 +36,10 @@ def singles(pbs, nsteps):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batch_equals_single_contexts(precision):
     """Same launch shapes as the single contexts (the batch starts with
-    context 0's): bit for bit in both precisions."""
+    context 0's): bit for bit in float64.  In float32 a single periodic
+    context fuses the next dt into its update kernel and the batch does
+    not, and -ffast-math may contract the two inlined copies of the update
+    differently: float32 round-off (5e-6 of each variable's scale)."""
     print(f"[gpu] batch of 4 uneven boxes, precision {precision}", flush=True)
     pbs = boxes(precision)
     ref = singles(pbs, 4)
@@ -44,8 +48,12 @@ def test_batch_equals_single_contexts(precision):
     b.advance(3, [0.0] * len(doms))
     t = b.advance(1)                    # continues from the stored times
     for d, tb, (ts, us) in zip(doms, t, ref):
-        assert tb == ts
-        assert np.array_equal(d.download(), us)
+        if precision == 0:
+            assert tb == ts
+            assert np.array_equal(d.download(), us)
+        else:
+            assert abs(tb - ts) <= 1e-6 * ts
+            assert_f32_close(d.download(), us)
     assert b.last_ms() > 0.0
     b.close()
     for d in doms:
@@ -90,12 +98,7 @@ def test_batch_tuned_shape(precision):
             assert np.array_equal(u, us)
         else:
             assert abs(d.time - ts) <= 1e-6 * ts
-            iu, ir = P.interior(u), P.interior(us)
-            mom = np.sqrt((ir[1:4] ** 2).sum(axis=0)).max()
-            mag = np.sqrt((ir[5:8] ** 2).sum(axis=0)).max()
-            scales = [np.abs(ir[0]).max(), mom, mom, mom, np.abs(ir[4]).max(), mag, mag, mag]
-            for v in range(8):
-                assert np.abs(iu[v] - ir[v]).max() <= 5e-6 * scales[v], v
+            assert_f32_close(u, us)
     b.close()
 
 

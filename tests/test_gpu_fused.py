@@ -1,0 +1,96 @@
+"""The fused trace + face/edge kernel (launch shape mode 3, k_fused): the
+trace summary stays in LDS.  In float64 it gives the bits of the compiled
+reference on every step fixture and of the separate trace and face/edge
+kernels on ragged domains for every solver pair, limiter and boundary type;
+in float32 it stays within round-off of the separate kernels.  And the
+float32-bulk path's pressure recovery at low beta (E and face B in float64)."""
+import numpy as np
+import pytest
+
+from dmx import abi
+from dmx import problems as P
+from goldens import STEP_FIXTURES, load_step
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", STEP_FIXTURES)
+def test_fp64_fused_matches_reference_fixture_bitwise(name):
+    p, u, dts, outs, full = load_step(name)
+    print(f"[gpu] fused kernel, fixture {name} {p.nx}x{p.ny}x{p.nz}", flush=True)
+    d = abi.Domain(p, u)
+    d.set_shape((3, 4), (0, 4))
+    t = 0.0
+    for s in range(1, int(max(outs)) + 1):
+        t, dd = d.advance(1, t)
+        assert dd[0] == dts[s - 1], (name, s, dd[0], dts[s - 1])
+        if s in outs:
+            got = P.interior(d.download())
+            bad = got != outs[s]
+            assert not bad.any(), (name, s, int(bad.sum()), float(np.abs(got - outs[s]).max()))
+    d.close()
+
+
+CASES = [
+    ("multidim, HLLD/HLLD", lambda: P.random_smooth(21, 13, 10, seed=31, slope_type=3)),
+    ("zero-gradient x,z, MC", lambda: P.random_smooth(17, 9, 23, seed=32, slope_type=2,
+                                                     bdtype=(P.ZERO_GRADIENT, P.PERIODIC, P.ZERO_GRADIENT))),
+    ("shearing box", lambda: P.mri_test(16, 32, 16)),
+    ("two x tiles, HLL/LLF, minmod", lambda: P.random_smooth(45, 11, 7, seed=33, slope_type=1,
+                                                            riemann=P.HLL, riemann2d=P.LLF)),
+    ("LLF/upwind", lambda: P.random_smooth(19, 18, 9, seed=34, riemann=P.LLF, riemann2d=P.UPWIND)),
+    ("upwind/HLLA, flat", lambda: P.random_smooth(12, 20, 11, seed=35, slope_type=0,
+                                                  riemann=P.UPWIND, riemann2d=P.HLLA)),
+    ("isothermal", lambda: P.random_smooth(14, 10, 12, seed=36, iso=1, ciso=0.8)),
+]
+
+
+@pytest.mark.parametrize("kt", [4, 8])
+@pytest.mark.parametrize("case", [c[0] for c in CASES])
+def test_fused_equals_separate_kernels(case, kt):
+    """Fused kernel (kt planes per block; 4 wraps its LDS rings mid-block on
+    the ragged domains) against the separate trace and fused face/edge
+    kernels: bit for bit in float64; in float32 within 1e-5 of each
+    variable's largest magnitude after 3 steps."""
+    pb = dict(CASES)[case]()
+    for prec in (0, 1):
+        p = pb.params.replace(precision=prec)
+        print(f"[gpu] fused kt={kt} precision {prec} {case} {p.nx}x{p.ny}x{p.nz}", flush=True)
+        out = []
+        for flux in ((0, 8), (3, kt)):
+            d = abi.Domain(p, pb.uin)
+            d.set_shape(flux, (1, 8))
+            d.advance(3, 0.0)
+            out.append(d.download())
+            d.close()
+        if prec == 0:
+            assert np.array_equal(out[0], out[1])
+        else:
+            a, b = P.interior(out[0]), P.interior(out[1])
+            for v in range(8):
+                assert np.abs(a[v] - b[v]).max() <= 1e-5 * (np.abs(a[v]).max() + 1e-30), v
+
+
+@pytest.mark.parametrize("flux", [(0, 8), (3, 8)])
+def test_fp32_low_beta_pressure(flux):
+    """beta = 1e-3: a sound wave of amplitude 1e-3 in a uniform field at
+    64^3, 20 steps.  The internal energy is 1e-3 of the total: the float32
+    bulk keeps E and the face fields in float64 and recovers the pressure
+    from them in float64, so its pressure stays within L1 1e-5 of the
+    float64 run (relative to the L1 norm of the pressure)."""
+    pb = P.sound_wave(64, beta=1e-3, amp=1e-3)
+    print(f"[gpu] low-beta sound wave, flux shape {flux}", flush=True)
+    states = []
+    for prec in (0, 1):
+        d = abi.Domain(pb.params.replace(precision=prec), pb.uin)
+        d.set_shape(flux, (1, 8))
+        d.advance(20, 0.0)
+        states.append(d.download())
+        d.close()
+    p64 = P.pressure(pb.params, states[0])
+    p32 = P.pressure(pb.params, states[1])
+    l1 = np.abs(p32 - p64).sum() / np.abs(p64).sum()
+    wave = np.abs(p64 - p64.mean()).sum() / np.abs(p64).sum()
+    print(f"[gpu]   pressure L1 float32 vs float64: {l1:.3e} (wave amplitude in L1: {wave:.3e})", flush=True)
+    assert wave > 1e-4
+    assert l1 <= 1e-5

@@ -250,8 +250,11 @@ def test_fused_update_dt_equals_separate_dt(monkeypatch):
     kernel (DMX_FUSE_DT=0) bit for bit, also when the run is split over
     several calls (each call's last update is not fused and brings the
     faces home); step-by-step calls and a batch then see the same state.
-    float32 (plain and learned limiter) does not fuse, so the switch leaves
-    it unchanged."""
+    float32 (plain and learned limiter) fuses too: -ffast-math may contract
+    the two inlined copies of the update differently, so the runs agree to
+    float32 round-off (dts to 1e-6, states to 5e-6 of each variable's
+    scale)."""
+    from goldens import assert_f32_close
     from dmx import learned as L
     cases = [(P.orszag_tang(16), 0), (P.random_smooth(21, 13, 10, seed=41, slope_type=2), 0),
              (P.random_smooth(45, 11, 7, seed=42, slope_type=3), 0), (P.orszag_tang(16), 1)]
@@ -269,8 +272,12 @@ def test_fused_update_dt_equals_separate_dt(monkeypatch):
             t, dts_c = d.advance(1)
             runs.append((np.concatenate([dts_a, dts_b, dts_c]), d.download(), d))
         (dts1, u1, d1), (dts0, u0, d0) = runs
-        assert np.array_equal(dts1, dts0)
-        assert np.array_equal(u1, u0)
+        if prec == 0:
+            assert np.array_equal(dts1, dts0)
+            assert np.array_equal(u1, u0)
+        else:
+            np.testing.assert_allclose(dts1, dts0, rtol=1e-6)
+            assert_f32_close(u1, u0)
         if prec == 0:
             # the state a step-by-step call and a batch see is the same
             for d in (d1, d0):
@@ -295,8 +302,8 @@ def test_fused_update_dt_equals_separate_dt(monkeypatch):
         _, dts = d.advance(5, 0.0)
         out.append((dts, d.download()))
         d.close()
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
+    np.testing.assert_allclose(out[0][0], out[1][0], rtol=1e-6)
+    assert_f32_close(out[0][1], out[1][1])
 
 
 def test_set_shape():
@@ -305,15 +312,15 @@ def test_set_shape():
     pb = P.random_smooth(20, 12, 14, seed=41)
     print("[gpu] set_shape", flush=True)
     out = []
-    for flux, trace in (((0, 8), (0, 4)), ((2, 16), (1, 5)), ((1, 4), (1, 32))):
+    for flux, trace in (((0, 8), (0, 4)), ((2, 16), (1, 5)), ((1, 4), (1, 32)), ((3, 8), (0, 4))):
         d = abi.Domain(pb.params, pb.uin)
         d.set_shape(flux, trace)
         d.advance(3, 0.0)
         out.append(d.download())
         d.close()
-    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+    assert all(np.array_equal(out[0], o) for o in out[1:])
     d = abi.Domain(pb.params, pb.uin)
-    for flux, trace in (((3, 8), (0, 4)), ((0, 5), (0, 4)), ((2, 4), (0, 4)), ((0, 8), (1, 0)), ((0, 8), (2, 4))):
+    for flux, trace in (((4, 8), (0, 4)), ((0, 5), (0, 4)), ((2, 4), (0, 4)), ((0, 8), (1, 0)), ((0, 8), (2, 4))):
         with pytest.raises(abi.DmxError):
             d.set_shape(flux, trace)
     d.close()
