@@ -58,25 +58,40 @@ __device__ __forceinline__ T cfast_b2(T gamma, T ploc, T r, T bn, T b2)
 // 1D Riemann solvers, solver.f90:393-935.
 // q = [r, p, u_n, B_n, u_t1, B_t1, u_t2, B_t2]; fg[8]; pstar = total pressure
 // of the sampled state (HLLD only; 0 for the others, as pgstar is zeroed).
+//
+// The energy chain -- total energies and pressures, v.B, the star-state
+// energies, the energy flux, and the total-pressure differences that set
+// the contact speed -- is evaluated in float64 (type D) in both precisions.
+// At low beta the magnetic energy dominates each of these terms and their
+// differences carry the thermal energy flux, which float32 would resolve
+// only to about eps_32 / beta.  In the float64 build D is T and every
+// expression keeps the reference's order, so the bits are unchanged.
 // ---------------------------------------------------------------------------
+typedef double D;
+
 template <typename T>
-__device__ __forceinline__ void phys_flux(T r, T ploc, T u, T v, T w, T a, T b, T c, T entho,
-                                          T* f, T* cons)
+__device__ __forceinline__ void phys_flux(T r, T ploc, T u, T v, T w, T a, T b, T c, D entho,
+                                          T* f, T* cons, D& fE, D& cE)
 {
     const T half = K<T>::half;
-    T ecin = half * ((u * u + v * v) + w * w) * r;
     T emag = half * ((a * a + b * b) + c * c);
-    T etot = (ploc * entho + ecin) + emag;
     T ptot = ploc + emag;
+    // energy chain in float64
+    D ecinD = 0.5 * ((D(u) * u + D(v) * v) + D(w) * w) * r;
+    D emagD = 0.5 * ((D(a) * a + D(b) * b) + D(c) * c);
+    D etotD = (D(ploc) * entho + ecinD) + emagD;
+    D ptotD = D(ploc) + emagD;
+    fE = (etotD + ptotD) * u - D(a) * ((D(u) * a + D(v) * b) + D(w) * c);
+    cE = etotD;
     f[0] = r * u;
-    f[1] = (etot + ptot) * u - a * ((u * a + v * b) + w * c);
+    f[1] = T(fE);
     f[2] = (r * u * u + ptot) - a * a;
     f[3] = K<T>::zero;
     f[4] = r * u * v - a * b;
     f[5] = b * u - a * v;
     f[6] = r * u * w - a * c;
     f[7] = c * u - a * w;
-    cons[0] = r; cons[1] = etot; cons[2] = r * u; cons[3] = a;
+    cons[0] = r; cons[1] = T(etotD); cons[2] = r * u; cons[3] = a;
     cons[4] = r * v; cons[5] = b; cons[6] = r * w; cons[7] = c;
 }
 
@@ -85,30 +100,31 @@ template <typename T>
 __device__ __forceinline__ void hlld1d(const T* ql, const T* qr, T gamma, T ploc, T proc,
                                        T* fg, T& pstar)
 {
-    const T zero = K<T>::zero, one = K<T>::one, half = K<T>::half;
+    const T zero = K<T>::zero, half = K<T>::half;
     T rl = ql[0], ul = ql[2], al = ql[3], vl = ql[4], bl = ql[5], wl = ql[6], cl = ql[7];
     T rr = qr[0], ur = qr[2], ar = qr[3], vr = qr[4], br = qr[5], wr = qr[6], cr = qr[7];
-    T entho = one / (gamma - one);
+    const D entho = 1.0 / (D(gamma) - 1.0);
     T a = half * (al + ar);
     T sgnm = sgn1(a);
-    T ecinl = half * ((ul * ul + vl * vl) + wl * wl) * rl;
-    T emagl = half * ((a * a + bl * bl) + cl * cl);
-    T etotl = (ploc * entho + ecinl) + emagl;
-    T ptotl = ploc + emagl;
-    T vdotbl = (ul * a + vl * bl) + wl * cl;
-    T ecinr = half * ((ur * ur + vr * vr) + wr * wr) * rr;
-    T emagr = half * ((a * a + br * br) + cr * cr);
-    T etotr = (proc * entho + ecinr) + emagr;
-    T ptotr = proc + emagr;
-    T vdotbr = (ur * a + vr * br) + wr * cr;
+    D ecinl = 0.5 * ((D(ul) * ul + D(vl) * vl) + D(wl) * wl) * rl;
+    D emagl = 0.5 * ((D(a) * a + D(bl) * bl) + D(cl) * cl);
+    D etotl = (D(ploc) * entho + ecinl) + emagl;
+    D ptotl = D(ploc) + emagl;
+    D vdotbl = (D(ul) * a + D(vl) * bl) + D(wl) * cl;
+    D ecinr = 0.5 * ((D(ur) * ur + D(vr) * vr) + D(wr) * wr) * rr;
+    D emagr = 0.5 * ((D(a) * a + D(br) * br) + D(cr) * cr);
+    D etotr = (D(proc) * entho + ecinr) + emagr;
+    D ptotr = D(proc) + emagr;
+    D vdotbr = (D(ur) * a + D(vr) * br) + D(wr) * cr;
     T cfastl = cfast_b2(gamma, ploc, rl, a, (a * a + bl * bl) + cl * cl);
     T cfastr = cfast_b2(gamma, proc, rr, a, (a * a + br * br) + cr * cr);
     T sl = fmn(ul, ur) - fmx(cfastl, cfastr);
     T sr = fmx(ul, ur) + fmx(cfastl, cfastr);
     T rcl = rl * (ul - sl);
     T rcr = rr * (sr - ur);
-    T ustar = ((rcr * ur + rcl * ul) + (ptotl - ptotr)) / (rcr + rcl);
-    T ptotstar = ((rcr * ptotl + rcl * ptotr) + rcl * rcr * (ul - ur)) / (rcr + rcl);
+    const D ustarD = ((D(rcr) * ur + D(rcl) * ul) + (ptotl - ptotr)) / (D(rcr) + rcl);
+    const T ustar = T(ustarD);
+    const D ptotstar = ((D(rcr) * ptotl + D(rcl) * ptotr) + D(rcl) * rcr * (D(ul) - ur)) / (D(rcr) + rcl);
     T vstarl, bstarl, wstarl, cstarl, vstarr, bstarr, wstarr, cstarr;
     T rstarl = rl * (sl - ul) / (sl - ustar);
     T estar = rl * (sl - ul) * (sl - ustar) - a * a;
@@ -121,8 +137,9 @@ __device__ __forceinline__ void hlld1d(const T* ql, const T* qr, T gamma, T ploc
         wstarl = wl - a * cl * (ustar - ul) / estar;
         cstarl = cl * el / estar;
     }
-    T vdotbstarl = (ustar * a + vstarl * bstarl) + wstarl * cstarl;
-    T etotstarl = ((((sl - ul) * etotl - ptotl * ul) + ptotstar * ustar) + a * (vdotbl - vdotbstarl)) / (sl - ustar);
+    D vdotbstarl = (ustarD * a + D(vstarl) * bstarl) + D(wstarl) * cstarl;
+    D etotstarl = ((((D(sl) - ul) * etotl - ptotl * ul) + ptotstar * ustarD) + D(a) * (vdotbl - vdotbstarl))
+                  / (D(sl) - ustarD);
     T sqrrstarl = sqrt(rstarl);
     T calfvenl = fab(a) / sqrrstarl;
     T sal = ustar - calfvenl;
@@ -137,8 +154,9 @@ __device__ __forceinline__ void hlld1d(const T* ql, const T* qr, T gamma, T ploc
         wstarr = wr - a * cr * (ustar - ur) / estar;
         cstarr = cr * er / estar;
     }
-    T vdotbstarr = (ustar * a + vstarr * bstarr) + wstarr * cstarr;
-    T etotstarr = ((((sr - ur) * etotr - ptotr * ur) + ptotstar * ustar) + a * (vdotbr - vdotbstarr)) / (sr - ustar);
+    D vdotbstarr = (ustarD * a + D(vstarr) * bstarr) + D(wstarr) * cstarr;
+    D etotstarr = ((((D(sr) - ur) * etotr - ptotr * ur) + ptotstar * ustarD) + D(a) * (vdotbr - vdotbstarr))
+                  / (D(sr) - ustarD);
     T sqrrstarr = sqrt(rstarr);
     T calfvenr = fab(a) / sqrrstarr;
     T sar = ustar + calfvenr;
@@ -147,46 +165,52 @@ __device__ __forceinline__ void hlld1d(const T* ql, const T* qr, T gamma, T ploc
     T wss = ((sqrrstarl * wstarl + sqrrstarr * wstarr) + sgnm * (cstarr - cstarl)) / den;
     T bss = ((sqrrstarl * bstarr + sqrrstarr * bstarl) + sgnm * sqrrstarl * sqrrstarr * (vstarr - vstarl)) / den;
     T css = ((sqrrstarl * cstarr + sqrrstarr * cstarl) + sgnm * sqrrstarl * sqrrstarr * (wstarr - wstarl)) / den;
-    T vdotbss = (ustar * a + vss * bss) + wss * css;
-    T etotssl = etotstarl - sgnm * sqrrstarl * (vdotbstarl - vdotbss);
-    T etotssr = etotstarr + sgnm * sqrrstarr * (vdotbstarr - vdotbss);
-    T ro, uo, vo, wo, bo, co, ptoto, etoto, vdotbo;
+    D vdotbss = (ustarD * a + D(vss) * bss) + D(wss) * css;
+    D etotssl = etotstarl - D(sgnm) * sqrrstarl * (vdotbstarl - vdotbss);
+    D etotssr = etotstarr + D(sgnm) * sqrrstarr * (vdotbstarr - vdotbss);
+    T ro, uo, vo, wo, bo, co;
+    D uoD, ptoto, etoto, vdotbo;
     if (sl > zero) {
-        ro = rl; uo = ul; vo = vl; wo = wl; bo = bl; co = cl; ptoto = ptotl; etoto = etotl; vdotbo = vdotbl;
+        ro = rl; uo = ul; vo = vl; wo = wl; bo = bl; co = cl; uoD = ul; ptoto = ptotl; etoto = etotl; vdotbo = vdotbl;
     } else if (sal > zero) {
-        ro = rstarl; uo = ustar; vo = vstarl; wo = wstarl; bo = bstarl; co = cstarl; ptoto = ptotstar; etoto = etotstarl; vdotbo = vdotbstarl;
+        ro = rstarl; uo = ustar; vo = vstarl; wo = wstarl; bo = bstarl; co = cstarl; uoD = ustarD; ptoto = ptotstar;
+        etoto = etotstarl; vdotbo = vdotbstarl;
     } else if (ustar > zero) {
-        ro = rstarl; uo = ustar; vo = vss; wo = wss; bo = bss; co = css; ptoto = ptotstar; etoto = etotssl; vdotbo = vdotbss;
+        ro = rstarl; uo = ustar; vo = vss; wo = wss; bo = bss; co = css; uoD = ustarD; ptoto = ptotstar;
+        etoto = etotssl; vdotbo = vdotbss;
     } else if (sar > zero) {
-        ro = rstarr; uo = ustar; vo = vss; wo = wss; bo = bss; co = css; ptoto = ptotstar; etoto = etotssr; vdotbo = vdotbss;
+        ro = rstarr; uo = ustar; vo = vss; wo = wss; bo = bss; co = css; uoD = ustarD; ptoto = ptotstar;
+        etoto = etotssr; vdotbo = vdotbss;
     } else if (sr > zero) {
-        ro = rstarr; uo = ustar; vo = vstarr; wo = wstarr; bo = bstarr; co = cstarr; ptoto = ptotstar; etoto = etotstarr; vdotbo = vdotbstarr;
+        ro = rstarr; uo = ustar; vo = vstarr; wo = wstarr; bo = bstarr; co = cstarr; uoD = ustarD; ptoto = ptotstar;
+        etoto = etotstarr; vdotbo = vdotbstarr;
     } else {
-        ro = rr; uo = ur; vo = vr; wo = wr; bo = br; co = cr; ptoto = ptotr; etoto = etotr; vdotbo = vdotbr;
+        ro = rr; uo = ur; vo = vr; wo = wr; bo = br; co = cr; uoD = ur; ptoto = ptotr; etoto = etotr; vdotbo = vdotbr;
     }
     fg[0] = ro * uo;
-    fg[1] = (etoto + ptoto) * uo - a * vdotbo;
+    fg[1] = T((etoto + ptoto) * uoD - D(a) * vdotbo);
     fg[2] = ro * uo * uo - a * a;
     fg[3] = zero;
     fg[4] = ro * uo * vo - a * bo;
     fg[5] = bo * uo - a * vo;
     fg[6] = ro * uo * wo - a * co;
     fg[7] = co * uo - a * wo;
-    pstar = ptoto;
+    pstar = T(ptoto);
 }
 
 // llf (solver.f90:393-498), hll (:504-612), upwind (:842-935)
 template <typename T, int SOLVER>
 __device__ __forceinline__ void simple1d(const T* ql, const T* qr, T gamma, T ploc, T proc, T* fg)
 {
-    const T zero = K<T>::zero, one = K<T>::one, half = K<T>::half;
+    const T zero = K<T>::zero, half = K<T>::half;
     T rl = ql[0], ul = ql[2], al = ql[3], vl = ql[4], bl = ql[5], wl = ql[6], cl = ql[7];
     T rr = qr[0], ur = qr[2], ar = qr[3], vr = qr[4], br = qr[5], wr = qr[6], cr = qr[7];
-    T entho = one / (gamma - one);
+    const D entho = 1.0 / (D(gamma) - 1.0);
     T a = half * (al + ar);
     T fl[8], cL[8], fr[8], cR[8];
-    phys_flux(rl, ploc, ul, vl, wl, a, bl, cl, entho, fl, cL);
-    phys_flux(rr, proc, ur, vr, wr, a, br, cr, entho, fr, cR);
+    D flE, cLE, frE, cRE;
+    phys_flux(rl, ploc, ul, vl, wl, a, bl, cl, entho, fl, cL, flE, cLE);
+    phys_flux(rr, proc, ur, vr, wr, a, br, cr, entho, fr, cR, frE, cRE);
     if (SOLVER == 3) { // hll
         T vleft = cfast_b2(gamma, ploc, rl, a, (a * a + bl * bl) + cl * cl);
         T vright = cfast_b2(gamma, proc, rr, a, (a * a + br * br) + cr * cr);
@@ -194,6 +218,7 @@ __device__ __forceinline__ void simple1d(const T* ql, const T* qr, T gamma, T pl
         T sr = fmx(fmx(ul, ur) + fmx(vleft, vright), zero);
 #pragma unroll
         for (int n = 0; n < 8; n++) fg[n] = ((sr * fl[n] - sl * fr[n]) + sr * sl * (cR[n] - cL[n])) / (sr - sl);
+        fg[1] = T(((D(sr) * flE - D(sl) * frE) + D(sr) * sl * (cRE - cLE)) / (D(sr) - sl));
     } else {
         T vel;
         if (SOLVER == 2) { // llf
@@ -205,6 +230,7 @@ __device__ __forceinline__ void simple1d(const T* ql, const T* qr, T gamma, T pl
         }
 #pragma unroll
         for (int n = 0; n < 8; n++) fg[n] = half * (fl[n] + fr[n]) - vel * half * (cR[n] - cL[n]);
+        fg[1] = T(0.5 * (flE + frE) - D(vel) * 0.5 * (cRE - cLE));
     }
 }
 
@@ -253,18 +279,20 @@ __device__ __forceinline__ T hlld2d(const T* LL, const T* RL, const T* LR, const
     T SR = fmx(fmx(fmx(uLL, uLR), uRL), uRR) + fmx(fmx(fmx(cLLx, cLRx), cRLx), cRRx);
     T SB = fmn(fmn(fmn(vLL, vLR), vRL), vRR) - fmx(fmx(fmx(cLLy, cLRy), cRLy), cRRy);
     T ST = fmx(fmx(fmx(vLL, vLR), vRL), vRR) + fmx(fmx(fmx(cLLy, cLRy), cRLy), cRRy);
-    T PtotLL = pLL + half * ((ALL * ALL + BLL * BLL) + CLL * CLL);
-    T PtotLR = pLR + half * ((ALR * ALR + BLR * BLR) + CLR * CLR);
-    T PtotRL = pRL + half * ((ARL * ARL + BRL * BRL) + CRL * CRL);
-    T PtotRR = pRR + half * ((ARR * ARR + BRR * BRR) + CRR * CRR);
+    // total pressures in float64 (D): their differences set the contact
+    // speeds, and at low beta the magnetic pressure dominates each of them
+    D PtotLL = D(pLL) + 0.5 * ((D(ALL) * ALL + D(BLL) * BLL) + D(CLL) * CLL);
+    D PtotLR = D(pLR) + 0.5 * ((D(ALR) * ALR + D(BLR) * BLR) + D(CLR) * CLR);
+    D PtotRL = D(pRL) + 0.5 * ((D(ARL) * ARL + D(BRL) * BRL) + D(CRL) * CRL);
+    D PtotRR = D(pRR) + 0.5 * ((D(ARR) * ARR + D(BRR) * BRR) + D(CRR) * CRR);
     T rcLLx = rLL * (uLL - SL), rcRLx = rRL * (SR - uRL);
     T rcLRx = rLR * (uLR - SL), rcRRx = rRR * (SR - uRR);
     T rcLLy = rLL * (vLL - SB), rcLRy = rLR * (ST - vLR);
     T rcRLy = rRL * (vRL - SB), rcRRy = rRR * (ST - vRR);
-    T ustar = ((((rcLLx * uLL + rcLRx * uLR) + rcRLx * uRL) + rcRRx * uRR)
-               + (((PtotLL - PtotRL) + PtotLR) - PtotRR)) / (((rcLLx + rcLRx) + rcRLx) + rcRRx);
-    T vstar = ((((rcLLy * vLL + rcLRy * vLR) + rcRLy * vRL) + rcRRy * vRR)
-               + (((PtotLL - PtotLR) + PtotRL) - PtotRR)) / (((rcLLy + rcLRy) + rcRLy) + rcRRy);
+    T ustar = T(((((D(rcLLx) * uLL + D(rcLRx) * uLR) + D(rcRLx) * uRL) + D(rcRRx) * uRR)
+                 + (((PtotLL - PtotRL) + PtotLR) - PtotRR)) / (((D(rcLLx) + rcLRx) + rcRLx) + rcRRx));
+    T vstar = T(((((D(rcLLy) * vLL + D(rcLRy) * vLR) + D(rcRLy) * vRL) + D(rcRRy) * vRR)
+                 + (((PtotLL - PtotLR) + PtotRL) - PtotRR)) / (((D(rcLLy) + rcLRy) + rcRLy) + rcRRy));
     T SLustar = one / (SL - ustar), SRustar = one / (SR - ustar);
     T SBvstar = one / (SB - vstar), STvstar = one / (ST - vstar);
 

@@ -59,7 +59,9 @@ extern "C" {
 
 /* arithmetic of the step */
 #define DMX_FP64  0   /* everything in float64 (the reference's arithmetic) */
-#define DMX_FP32  1   /* bulk float32; float64 face-B/CT, pressure recovery, sums */
+#define DMX_FP32  1   /* bulk float32; float64 E, face B and CT, pressure
+                         recovery, the Riemann solvers' energy chain, dt
+                         and history sums */
 
 /* error codes */
 #define DMX_OK          0
@@ -167,22 +169,26 @@ int  dmx_profile(dmx_ctx* ctx, int32_t nsteps, double* time, double* ms, int32_t
 const char* dmx_kernel_name(int32_t id);
 
 /* Launch-shape autotuner: on the uploaded state, times the candidate shapes
- * of the trace -- mode 0: a 16x4x4 cell tile per block; mode 1: a 16x16
+ * of the trace -- mode 0: a 16x4x4 cell tile per block; mode 1: a 32x8
  * column tile marching 8, 16 or 32 z-planes -- and of the face/edge Riemann
  * stage -- mode 0: one fused kernel marching 4, 8, 16 or 32 z-planes per
  * block; mode 1: separate face and edge kernels; mode 2: the fused kernel
  * carrying the previous plane's states in LDS (8, 16 or 32 planes) -- keeps
- * the fastest of each for this context, and returns the face/edge stage's
- * time per launch in *best_ms and {face/edge mode, planes, trace mode,
- * planes} in shape[4] (either may be NULL).  Every shape gives the same
- * results in float64.  The state is unchanged.  No counterpart in the
- * reference's step: it takes the role of utils/pyutils/gpuconfig/gpuconfig.py,
- * which sweeps OpenACC vector sizes by re-running the code. */
+ * the fastest of each, then times mode 3, one kernel for the trace and the
+ * face/edge stage together (the trace summary stays in LDS; 8, 16 or 32
+ * planes), against the best separate pair and keeps it if faster.  Returns
+ * the face/edge stage's time per launch (mode 3: trace included) in
+ * *best_ms and {face/edge mode, planes, trace mode, planes} in shape[4]
+ * (either may be NULL).  Every shape gives the same results in float64.
+ * The state is unchanged.  No counterpart in the reference's step: it takes
+ * the role of utils/pyutils/gpuconfig/gpuconfig.py, which sweeps OpenACC
+ * vector sizes by re-running the code. */
 int  dmx_tune(dmx_ctx* ctx, double* best_ms, int32_t* shape);
 /* Sets the launch shapes without timing them: shape[4] as dmx_tune returns
- * it (face/edge mode 0..2 with 4, 8, 16 or 32 planes, mode 2 not with 4;
- * trace mode 0, or 1 with 1..64 planes); for runs that must repeat a shape
- * tuned before.  No counterpart in the reference. */
+ * it (face/edge mode 0..3 with 4, 8, 16 or 32 planes, mode 2 not with 4;
+ * trace mode 0, or 1 with 1..64 planes; mode 3 makes the trace shape
+ * unused); for runs that must repeat a shape tuned before.  No counterpart
+ * in the reference. */
 int  dmx_set_shape(dmx_ctx* ctx, const int32_t* shape);
 
 /* ---- batched ensemble -------------------------------------------------
