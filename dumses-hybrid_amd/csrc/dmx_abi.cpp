@@ -135,6 +135,17 @@ int dmx_compute_dt(dmx_ctx* c, double* dt)
     if (!dt) { dmx::set_error("null dt"); return DMX_E_PARAM; }
     return c->e->compute_dt(dt);
 }
+int dmx_compute_dt_noalloc(const dmx_params* p, const double* uin, double* dt, void* stream)
+{
+    int rc = validate(p);
+    if (rc) return rc;
+    if (p->precision != DMX_FP64) {
+        dmx::set_error("dmx_compute_dt_noalloc reads a float64 state: precision must be DMX_FP64");
+        return DMX_E_PARAM;
+    }
+    if (!uin || !dt) { dmx::set_error("null uin or dt"); return DMX_E_PARAM; }
+    return dmx::noalloc_compute_dt_f64(*p, uin, dt, stream);
+}
 int dmx_godunov(dmx_ctx* c, double time, double dt)
 {
     CTX_CHECK(c);

@@ -56,5 +56,8 @@ Engine* make_engine_f64(const dmx_params& p, int* rc);
 Engine* make_engine_f32(const dmx_params& p, int* rc);
 Batch* make_batch_f64(Engine* const* es, int n, int nstreams, int* rc);
 Batch* make_batch_f32(Engine* const* es, int n, int nstreams, int* rc);
+// dmx_compute_dt_noalloc (float64 state in the caller's device memory;
+// stream is a hipStream_t)
+int noalloc_compute_dt_f64(const dmx_params& p, const double* uin, double* dt, void* stream);
 
 } // namespace dmx

@@ -50,6 +50,7 @@ def params_to_c(p: Params) -> DmxParams:
 
 # every entry point include/dmx.h declares
 EXPORTS = ("dmx_create", "dmx_destroy", "dmx_upload", "dmx_download", "dmx_compute_dt",
+           "dmx_compute_dt_noalloc",
            "dmx_godunov", "dmx_fargo", "dmx_boundary", "dmx_dissipation", "dmx_advance", "dmx_history",
            "dmx_history_shear", "dmx_write_dump", "dmx_read_dump",
            "dmx_last_advance_ms", "dmx_profile", "dmx_kernel_name", "dmx_tune", "dmx_set_shape",
@@ -83,6 +84,7 @@ def lib(path: str | None = None):
     L.dmx_upload.argtypes = [vp, dp]
     L.dmx_download.argtypes = [vp, dp]
     L.dmx_compute_dt.argtypes = [vp, dp]
+    L.dmx_compute_dt_noalloc.argtypes = [ctypes.POINTER(DmxParams), vp, vp, vp]
     L.dmx_godunov.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.dmx_fargo.argtypes = [vp, ctypes.c_double]
     L.dmx_boundary.argtypes = [vp, ctypes.c_double, ctypes.c_double]

@@ -104,6 +104,21 @@ int  dmx_download(dmx_ctx* ctx, double* uin);
  * interior (src/subroutines/physics.f90:18-132). */
 int  dmx_compute_dt(dmx_ctx* ctx, double* dt);
 
+/* CFL timestep with no device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops or harnesses that forbid hipMalloc).  The same
+ * arithmetic as dmx_compute_dt, on a float64 state the caller already holds
+ * in device memory: uin is a DEVICE pointer to the reference layout above
+ * (the cell field is the mean of the left and right face fields, variables
+ * 6..8 and 9..11); *dt, a DEVICE pointer, receives the timestep.  The two
+ * kernels are enqueued on `stream` (a hipStream_t; NULL: the null stream)
+ * and the call returns without waiting.  p->precision must be DMX_FP64.
+ * The block minima go to a static 16 KiB device array (2048 doubles) that
+ * every call on a device shares: any domain size fits (the first kernel
+ * strides over the cells with at most 2048 blocks), and calls that overlap
+ * on one device race. */
+int  dmx_compute_dt_noalloc(const dmx_params* p, const double* uin, double* dt, void* stream);
+
 /* One Godunov step of length dt at time `time` (time enters the shearing-box
  * flux/EMF remap at time + dt/2, src/subroutines/scheme.f90:176-182). */
 int  dmx_godunov(dmx_ctx* ctx, double time, double dt);
