@@ -69,22 +69,26 @@ def stage_bytes(group: str, wt: int, wb: int = 8) -> float:
 # the launch shapes the autotuner keeps at 256^3.  Constants copied from
 # that file; COUNTERS_SOURCE names the file and the commit whose kernels it
 # measured.
-COUNTERS_SOURCE = "profiles/mhd_step_ot256_hbm_bytes.md, profiles/mhd_step_ot256_sq_counters.md (kernels of commit 8a235bc)"
+COUNTERS_SOURCE = ("profiles/mhd_step_ot256_hbm_bytes.md, profiles/mhd_step_ot256_sq_counters.md "
+                   "(kernels of commit 3e6fdd9)")
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 571.0, "flux": 1653.1, "update_ct": 416.8, "compute_dt": 86.0,
+    "f64": {"trace": 571.1, "flux": 1927.2, "update_ct": 416.5, "compute_dt": 86.1,
             "boundary": 17.7},
-    "f32": {"trace": 348.2, "flux": 837.6, "update_ct": 248.2, "compute_dt": 64.8,
-            "boundary": 14.1},
+    "f32": {"trace": 335.8, "flux": 831.1, "update_ct": 291.4, "compute_dt": 69.5,
+            "boundary": 14.8},
 }
 
 
 # VALU-busy fraction of each kernel group at 256^3 (SQ_ACTIVE_INST_VALU x 4
-# / 1024 SIMDs / kernel cycles) from the rocprofv3 SQ counter run summarised
-# in profiles/mhd_step_ot256_sq_counters.md: the bound that applies where the
-# HBM fraction is low (bit-exact float64 division and square root)
+# / 1024 SIMDs / kernel cycles) from the rocprofv3 SQ counter runs summarised
+# in profiles/mhd_step_ot256_sq_counters.md and profiles/fused_trace_flux_ot256.md:
+# the bound that applies where the HBM fraction is low (bit-exact float64
+# division and square root)
 MEASURED_VALU_BUSY = {
-    "f64": {"trace": 0.449, "flux": 0.699, "update_ct": 0.384, "compute_dt": 0.704, "boundary": 0.046},
-    "f32": {"trace": 0.405, "flux": 0.562, "update_ct": 0.126, "compute_dt": 0.414, "boundary": 0.055},
+    "f64": {"trace": 0.449, "flux": 0.707, "update_ct": 0.384, "compute_dt": 0.701, "boundary": 0.047,
+            "fused": 0.272},
+    "f32": {"trace": 0.411, "flux": 0.674, "update_ct": 0.253, "compute_dt": 0.414, "boundary": 0.058,
+            "fused": 0.358},
 }
 
 
@@ -95,6 +99,10 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
     the library's stream), against the 8 TB/s HBM peak."""
     if not prof:
         return None
+    prof = dict(prof)
+    if prof.get("faces", 0.0) > 0.0:
+        # separate face and edge kernels (launch mode 1): one face/edge stage
+        prof["flux"] = prof.pop("faces") + prof.pop("edges")
     dom = max(prof, key=prof.get)
     ms = prof[dom]
     b = ALG_BYTES_PER_ZONE_CYCLE[prec]
@@ -114,13 +122,16 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
 # planes), (trace mode, planes)); --no-tune applies them without timing, so
 # that profiler runs measure the kernels of the timed run (under a profiler
 # the autotuner's timings, and so its choice, can change)
-TUNED_256 = {0: ((0, 8), (1, 16)), 1: ((2, 8), (1, 8))}
+TUNED_256 = {0: ((0, 8), (1, 16)), 1: ((2, 8), (1, 32))}
+# --fused: the trace and the face/edge stage in one kernel (mode 3, the trace
+# summary kept in LDS), 16 planes per block, in place of the tuned pair
+FUSED_SHAPE = ((3, 16), (1, 16))
 
 
 REPEATS = 5
 
 
-def run_gpu(pb, steps, warmup, precision, limiter=None, tune=True, repeats=REPEATS):
+def run_gpu(pb, steps, warmup, precision, limiter=None, tune=True, repeats=REPEATS, shape=None):
     """Warm-up, then `repeats` timed regions of `steps` steps each of the
     instrumented advance (HIP events around every kernel group on the
     library's stream, read once after the run), each bracketed by device
@@ -133,7 +144,11 @@ def run_gpu(pb, steps, warmup, precision, limiter=None, tune=True, repeats=REPEA
     d = abi.Domain(p, pb.uin)
     if limiter is not None:
         d.set_limiter(limiter)
-    if tune:
+    if shape is not None:
+        d.set_shape(*shape)
+        tuned_ms = None
+        log(f"[bench] fixed shapes: face/edge {d.shape}, trace {d.trace_shape}")
+    elif tune:
         tuned_ms = d.tune()                # launch-shape autotuner (state unchanged)
         log(f"[bench] tuned face/edge stage: mode {d.shape[0]}, {d.shape[1]} planes, {tuned_ms:.3f} ms; "
             f"trace: mode {d.trace_shape[0]}, {d.trace_shape[1]} planes")
@@ -267,6 +282,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="apply the launch shapes the autotuner keeps at 256^3 instead of tuning")
+    ap.add_argument("--fused", action="store_true",
+                    help="run the trace and the face/edge stage as one kernel (launch mode 3) "
+                         "instead of the tuned pair")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) to calibrate HBM byte counters")
     a = ap.parse_args()
@@ -297,14 +315,15 @@ def main():
         workload = f"3D Orszag-Tang {n}^3 periodic, HLLD/HLLD, MC slopes, CFL 0.7"
     cells = pb.params.nx * pb.params.ny * pb.params.nz
     log("[bench] float64 run")
-    r64 = run_gpu(pb, a.steps, a.warmup, 0, tune=not a.no_tune, repeats=a.repeats)
+    fixed = FUSED_SHAPE if a.fused else None
+    r64 = run_gpu(pb, a.steps, a.warmup, 0, tune=not a.no_tune, repeats=a.repeats, shape=fixed)
     v64 = cells * a.steps / r64["wall"]
     log(f"[bench] float64: {v64:.4g} zone-cycles/s, {1e3 * r64['wall'] / a.steps:.3f} ms/step "
         f"(median of {a.repeats})")
     out = {}
     if not a.no_fp32:
         log("[bench] float32-bulk run")
-        r32 = run_gpu(pb, a.steps, a.warmup, 1, tune=not a.no_tune, repeats=a.repeats)
+        r32 = run_gpu(pb, a.steps, a.warmup, 1, tune=not a.no_tune, repeats=a.repeats, shape=fixed)
         v32 = cells * a.steps / r32["wall"]
         u64, u32 = P.interior(r64["state"]), P.interior(r32["state"])
         names = ["rho", "rhou", "rhov", "rhow", "E", "Bx", "By", "Bz"]
@@ -324,7 +343,7 @@ def main():
         if os.path.exists(TL.WEIGHTS):
             log("[bench] float32-bulk run with the learned limiter")
             theta = np.load(TL.WEIGHTS)["theta"]
-            rl = run_gpu(pb, a.steps, a.warmup, 1, limiter=theta, tune=not a.no_tune, repeats=1)
+            rl = run_gpu(pb, a.steps, a.warmup, 1, limiter=theta, tune=not a.no_tune, repeats=1, shape=fixed)
             vl = cells * a.steps / rl["wall"]
             ul = P.interior(rl["state"])
             l1l = {names[v]: float(np.abs(ul[v] - u64[v]).sum() / norms[v]) for v in range(8)}
