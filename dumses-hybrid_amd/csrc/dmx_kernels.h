@@ -10,8 +10,9 @@
 //        from it; B: the CT face-B accumulation stays float64 in both
 //        precisions).  The reference's right faces are implied:
 //        B_R(i) = B_L(i+1).
-//   S  : 47 planes of T, the per-cell trace summary written by the trace
-//        kernel for cells [0, n+1]^3 (see TraceSlot).
+//   S  : 47 values of T per cell, the per-cell trace summary written by the
+//        trace kernel for cells [0, n+1]^3 (see TraceSlot), in groups of
+//        256 bytes of consecutive cells per slot (s_off in dmx_step.inc).
 //   F  : 18 planes of T, x/y/z face fluxes of rho, rho*u, rho*v, rho*w, E
 //        (already multiplied by ds*dt as compflux does) and the HLLD star
 //        total pressure p_T (raw).
