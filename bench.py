@@ -199,14 +199,14 @@ def run_ensemble(steps=50, warmup=2, nboxes=48):
     t0 = time.perf_counter()
     b.advance(steps)
     bat = cells * steps / (time.perf_counter() - t0)
-    shape = b.shape
+    shape, tshape = b.shape, b.trace_shape
     b.close()
     for d in doms:
         d.close()
     return dict(workload=f"{nboxes} periodic turbulence boxes, sizes from {{32,48,64,96,128}}^3 "
                          f"(seed 4242), float32 bulk, HLLD/HLLD, MC, {steps} steps",
                 cells=cells, batched=bat, per_context=seq, speedup=bat / seq, unit="zone-cycles/s",
-                shape=list(shape))
+                shape=list(shape), trace_shape=list(tshape))
 
 
 def cpu_baseline(n=128, steps=3, threads=(1, 2, 4, 8, 16)):

@@ -341,6 +341,7 @@ class Batch:
         _check(lib().dmx_batch_create(arr, len(self.domains), nstreams, ctypes.byref(h)), "dmx_batch_create")
         self.h = h
         self.shape = None
+        self.trace_shape = None
 
     def advance(self, nsteps: int, times=None) -> np.ndarray:
         """nsteps steps of every box from times (default: each domain's
@@ -356,6 +357,7 @@ class Batch:
         sh = (ctypes.c_int32 * 4)()
         _check(lib().dmx_batch_tune(self.h, ctypes.byref(ms), sh), "dmx_batch_tune")
         self.shape = (int(sh[0]), int(sh[1]))
+        self.trace_shape = (int(sh[2]), int(sh[3]))
         return ms.value
 
     def last_ms(self) -> float:

@@ -226,9 +226,12 @@ void dmx_batch_destroy(dmx_batch* batch);
 /* nsteps steps of every box; times[n] holds each box's start time on entry
  * and its end time on return */
 int  dmx_batch_advance(dmx_batch* batch, int32_t nsteps, double* times);
-/* launch-shape autotuner of the batched face/edge stage (fused or carrying
+/* launch-shape autotuner of the batch: the trace (the 16x4x4 tile, or the
+ * z-march over 8/16/32 planes) and the face/edge stage (fused or carrying
  * kernel, 4/8/16/32 planes per block); state unchanged, shape[4] as
- * dmx_tune (the batch traces with the 16x4x4 tile: trace mode 0) */
+ * dmx_tune.  A batch starts with context 0's shapes (a z-march over another
+ * plane count takes the nearest of 8, 16, 32; face/edge modes 1 and 3 take
+ * mode 0). */
 int  dmx_batch_tune(dmx_batch* batch, double* best_ms, int32_t* shape);
 /* milliseconds of the last dmx_batch_advance on the device */
 int  dmx_batch_last_ms(dmx_batch* batch, double* ms);

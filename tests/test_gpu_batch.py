@@ -56,6 +56,34 @@ def test_batch_equals_single_contexts(precision):
         d.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("kt", [8, 16, 32])
+def test_batch_z_march_trace(precision, kt):
+    """The batch's z-marching trace (context 0 set to it before the batch
+    is made) against single contexts run with the same trace and face/edge
+    shapes: bit for bit in both precisions."""
+    print(f"[gpu] batch with the z-march trace over {kt} planes, precision {precision}", flush=True)
+    pbs = boxes(precision)
+    shape = ((0, 8), (1, kt))
+    ref = []
+    for pb in pbs:
+        d = abi.Domain(pb.params, pb.uin)
+        d.set_shape(*shape)
+        t, _ = d.advance(4, 0.0)
+        ref.append((t, d.download()))
+        d.close()
+    doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    doms[0].set_shape(*shape)
+    b = abi.Batch(doms)
+    t = b.advance(4, [0.0] * len(doms))
+    for d, tb, (ts, us) in zip(doms, t, ref):
+        assert tb == ts
+        assert np.array_equal(d.download(), us)
+    b.close()
+    for d in doms:
+        d.close()
+
+
 @pytest.mark.parametrize("nstreams", [1, 3, 8])
 def test_batch_streams(nstreams):
     """The boxes dealt to 1, 3 or 8 (capped at the box count) streams: the
@@ -75,8 +103,8 @@ def test_batch_streams(nstreams):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batch_tuned_shape(precision):
     """After the batch autotuner (state unchanged): float64 bit for bit; in
-    float32 the kept face/edge kernel may differ from the single contexts'
-    and contract its arithmetic differently, so the states agree to float32
+    float32 the kept trace and face/edge kernels may differ from the single
+    contexts' and contract their arithmetic differently, so the states agree to float32
     rounding: 5e-6 of the largest magnitude of the variable (rho, E) or of
     its vector (momentum, B) after 4 steps."""
     print(f"[gpu] tuned batch, precision {precision}", flush=True)
