@@ -70,11 +70,11 @@ def stage_bytes(group: str, wt: int, wb: int = 8) -> float:
 # that file; COUNTERS_SOURCE names the file and the commit whose kernels it
 # measured.
 COUNTERS_SOURCE = ("profiles/mhd_step_ot256_hbm_bytes.md, profiles/mhd_step_ot256_sq_counters.md "
-                   "(kernels of commit 3e6fdd9)")
+                   "(kernels of commit 8ab5e83)")
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 571.1, "flux": 1927.2, "update_ct": 416.5, "compute_dt": 86.1,
+    "f64": {"trace": 570.7, "flux": 1661.8, "update_ct": 416.8, "compute_dt": 86.1,
             "boundary": 17.7},
-    "f32": {"trace": 335.8, "flux": 831.1, "update_ct": 291.4, "compute_dt": 69.5,
+    "f32": {"trace": 333.7, "flux": 839.5, "update_ct": 291.5, "compute_dt": 69.5,
             "boundary": 14.8},
 }
 
@@ -85,9 +85,9 @@ MEASURED_TRAFFIC = {
 # the bound that applies where the HBM fraction is low (bit-exact float64
 # division and square root)
 MEASURED_VALU_BUSY = {
-    "f64": {"trace": 0.449, "flux": 0.707, "update_ct": 0.384, "compute_dt": 0.701, "boundary": 0.047,
+    "f64": {"trace": 0.505, "flux": 0.725, "update_ct": 0.383, "compute_dt": 0.734, "boundary": 0.048,
             "fused": 0.272},
-    "f32": {"trace": 0.411, "flux": 0.674, "update_ct": 0.253, "compute_dt": 0.414, "boundary": 0.058,
+    "f32": {"trace": 0.413, "flux": 0.640, "update_ct": 0.254, "compute_dt": 0.425, "boundary": 0.055,
             "fused": 0.358},
 }
 
