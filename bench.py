@@ -122,7 +122,7 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
 # planes), (trace mode, planes)); --no-tune applies them without timing, so
 # that profiler runs measure the kernels of the timed run (under a profiler
 # the autotuner's timings, and so its choice, can change)
-TUNED_256 = {0: ((0, 8), (1, 16)), 1: ((2, 8), (1, 32))}
+TUNED_256 = {0: ((0, 8), (1, 16)), 1: ((3, 16), (1, 32))}
 # --fused: the trace and the face/edge stage in one kernel (mode 3, the trace
 # summary kept in LDS), 16 planes per block, in place of the tuned pair
 FUSED_SHAPE = ((3, 16), (1, 16))
