@@ -69,13 +69,12 @@ def stage_bytes(group: str, wt: int, wb: int = 8) -> float:
 # the launch shapes the autotuner keeps at 256^3.  Constants copied from
 # that file; COUNTERS_SOURCE names the file and the commit whose kernels it
 # measured.
-COUNTERS_SOURCE = ("profiles/mhd_step_ot256_hbm_bytes.md, profiles/mhd_step_ot256_sq_counters.md "
-                   "(kernels of commit 8ab5e83)")
+COUNTERS_SOURCE = ("profiles/mhd_step_ot256_hbm_bytes.md, profiles/mhd_step_ot256_sq_counters.md, "
+                   "profiles/fused_trace_flux_ot256.md (kernels of commit 6a78c52)")
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 570.7, "flux": 1661.8, "update_ct": 416.8, "compute_dt": 86.1,
+    "f64": {"trace": 570.8, "flux": 1659.9, "update_ct": 416.8, "compute_dt": 86.0,
             "boundary": 17.7},
-    "f32": {"trace": 333.7, "flux": 839.5, "update_ct": 291.5, "compute_dt": 69.5,
-            "boundary": 14.8},
+    "f32": {"fused": 235.8, "update_ct": 291.6, "compute_dt": 69.5, "boundary": 14.8},
 }
 
 
@@ -85,10 +84,10 @@ MEASURED_TRAFFIC = {
 # the bound that applies where the HBM fraction is low (bit-exact float64
 # division and square root)
 MEASURED_VALU_BUSY = {
-    "f64": {"trace": 0.505, "flux": 0.725, "update_ct": 0.383, "compute_dt": 0.734, "boundary": 0.048,
-            "fused": 0.272},
-    "f32": {"trace": 0.413, "flux": 0.640, "update_ct": 0.254, "compute_dt": 0.425, "boundary": 0.055,
-            "fused": 0.358},
+    "f64": {"trace": 0.505, "flux": 0.720, "update_ct": 0.391, "compute_dt": 0.723, "boundary": 0.048,
+            "fused": 0.511},
+    "f32": {"trace": 0.413, "flux": 0.640, "update_ct": 0.243, "compute_dt": 0.390, "boundary": 0.056,
+            "fused": 0.598},
 }
 
 
