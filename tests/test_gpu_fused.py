@@ -70,6 +70,32 @@ def test_fused_equals_separate_kernels(case, kt):
             assert_f32_close(out[1], out[0], rtol=1e-5)
 
 
+def test_fused_learned_limiter():
+    """The fused kernel with the learned limiter (slope_type 4: the trace
+    part reads the network's slopes) against the separate kernels: bit for
+    bit in float64 over 2 steps; in float32 within round-off after 1 step
+    (after that a round-off difference in a nearly uniform variable can move
+    a cell between the MC and the network branch)."""
+    from dmx import learned as L
+    pb = P.orszag_tang(24)
+    theta = 0.5 * L.init_params(9)
+    for prec in (0, 1):
+        p = pb.params.replace(precision=prec, slope_type=4)
+        print(f"[gpu] fused kernel, learned limiter, precision {prec}", flush=True)
+        out = []
+        for flux in ((0, 8), (3, 8)):
+            d = abi.Domain(p, pb.uin)
+            d.set_limiter(theta)
+            d.set_shape(flux, (1, 8))
+            d.advance(2 if prec == 0 else 1, 0.0)
+            out.append(d.download())
+            d.close()
+        if prec == 0:
+            assert np.array_equal(out[0], out[1])
+        else:
+            assert_f32_close(out[1], out[0], rtol=1e-5)
+
+
 @pytest.mark.parametrize("flux", [(0, 8), (3, 8)])
 @pytest.mark.parametrize("beta,bound", [(1e-2, 1e-5), (1e-3, 3e-5)])
 def test_fp32_low_beta_pressure(flux, beta, bound):
