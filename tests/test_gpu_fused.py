@@ -42,6 +42,9 @@ CASES = [
     ("upwind/HLLA, flat", lambda: P.random_smooth(12, 20, 11, seed=35, slope_type=0,
                                                   riemann=P.UPWIND, riemann2d=P.HLLA)),
     ("isothermal", lambda: P.random_smooth(14, 10, 12, seed=36, iso=1, ciso=0.8)),
+    # 65 faces a row: a 32- and a 33-column tile in float32 (7 rows), three
+    # of 16 and one of 17 in float64; 24 rows cut into 8s or 6s
+    ("wide and narrow tiles", lambda: P.random_smooth(64, 23, 9, seed=37)),
 ]
 
 
