@@ -227,11 +227,13 @@ void dmx_batch_destroy(dmx_batch* batch);
  * and its end time on return */
 int  dmx_batch_advance(dmx_batch* batch, int32_t nsteps, double* times);
 /* launch-shape autotuner of the batch: the trace (the 16x4x4 tile, or the
- * z-march over 8/16/32 planes) and the face/edge stage (fused or carrying
- * kernel, 4/8/16/32 planes per block); state unchanged, shape[4] as
- * dmx_tune.  A batch starts with context 0's shapes (a z-march over another
- * plane count takes the nearest of 8, 16, 32; face/edge modes 1 and 3 take
- * mode 0). */
+ * z-march over 8/16/32 planes), then the face/edge stage (fused or carrying
+ * kernel, 4/8/16/32 planes per block) after that trace against the trace
+ * and face/edge stage in one kernel (mode 3, 8/16/32 planes); state
+ * unchanged; best_ms is the chosen stage kernel's time, shape[4] as
+ * dmx_tune.  A batch starts with context 0's shapes (a z-march or mode 3
+ * over another plane count takes the nearest of 8, 16, 32; face/edge mode
+ * 1 takes mode 0). */
 int  dmx_batch_tune(dmx_batch* batch, double* best_ms, int32_t* shape);
 /* milliseconds of the last dmx_batch_advance on the device */
 int  dmx_batch_last_ms(dmx_batch* batch, double* ms);

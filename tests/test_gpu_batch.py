@@ -84,6 +84,35 @@ def test_batch_z_march_trace(precision, kt):
         d.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("kt", [8, 16, 32])
+def test_batch_fused_kernel(precision, kt):
+    """The batch's fused trace + face/edge kernel (kb_fused, the body of
+    k_fused over every box's tiles; context 0 set to it before the batch is
+    made) against single contexts run with the same shape: bit for bit in
+    both precisions."""
+    print(f"[gpu] batch with the fused kernel over {kt} planes, precision {precision}", flush=True)
+    pbs = boxes(precision)
+    shape = ((3, kt), (1, 16))
+    ref = []
+    for pb in pbs:
+        d = abi.Domain(pb.params, pb.uin)
+        d.set_shape(*shape)
+        t, _ = d.advance(4, 0.0)
+        ref.append((t, d.download()))
+        d.close()
+    doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    doms[0].set_shape(*shape)
+    b = abi.Batch(doms)
+    t = b.advance(4, [0.0] * len(doms))
+    for d, tb, (ts, us) in zip(doms, t, ref):
+        assert tb == ts
+        assert np.array_equal(d.download(), us)
+    b.close()
+    for d in doms:
+        d.close()
+
+
 @pytest.mark.parametrize("nstreams", [1, 3, 8])
 def test_batch_streams(nstreams):
     """The boxes dealt to 1, 3 or 8 (capped at the box count) streams: the
