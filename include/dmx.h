@@ -219,7 +219,9 @@ int  dmx_set_shape(dmx_ctx* ctx, const int32_t* shape);
  * are not used concurrently with it; their state stays in them (download
  * each with dmx_download).  The boxes are dealt, by decreasing size, to
  * nstreams groups (0: 2, at most 8), each launched on a stream of its own,
- * so that kernels of different kinds of the groups overlap on the GPU. */
+ * so that kernels of different kinds of the groups overlap on the GPU.  A
+ * group whose boxes are all periodic (and free of what dmx_advance cannot
+ * fuse the dt with) fuses the next dt into its update, as dmx_advance does. */
 typedef struct dmx_batch dmx_batch;
 int  dmx_batch_create(dmx_ctx* const* ctxs, int32_t n, int32_t nstreams, dmx_batch** batch);
 void dmx_batch_destroy(dmx_batch* batch);

@@ -113,6 +113,31 @@ def test_batch_fused_kernel(precision, kt):
         d.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nstreams", [1, 2])
+def test_batch_fused_dt(precision, nstreams):
+    """All boxes periodic: the batch fuses the next dt into its update
+    (kb_update_dt, the faces alternating between each box's two buffers)
+    over calls of 3 and 2 steps, and gives the bits and times of single
+    contexts; the faces end in their home buffers (the contexts advance
+    on alone afterwards and stay equal)."""
+    print(f"[gpu] batch with the fused dt, precision {precision}, {nstreams} stream(s)", flush=True)
+    pbs = [P.turbulence(16, seed=1, mach=1.0, beta=1.0, kmax=4, precision=precision),
+           P.random_smooth(24, 20, 12, seed=2, slope_type=1, precision=precision),
+           P.orszag_tang(8, precision=precision)]
+    ref = singles(pbs, 7)
+    doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    b = abi.Batch(doms, nstreams=nstreams)
+    b.advance(3, [0.0] * len(doms))
+    b.advance(2)
+    b.close()
+    for d, (ts, us) in zip(doms, ref):
+        d.advance(2)
+        assert d.time == ts
+        assert np.array_equal(d.download(), us)
+        d.close()
+
+
 @pytest.mark.parametrize("nstreams", [1, 3, 8])
 def test_batch_streams(nstreams):
     """The boxes dealt to 1, 3 or 8 (capped at the box count) streams: the
