@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dmx_tiling.h"
+
 namespace dmx {
 
 enum TraceSlot {
