@@ -353,6 +353,11 @@ class Batch:
         return t
 
     def tune(self) -> float:
+        """Autotunes the batch's launch shapes on the current states
+        (dmx_batch_tune: the trace, then the face/edge stage after it
+        against the fused trace + face/edge kernel, mode 3); keeps them in
+        self.shape and self.trace_shape and returns the chosen stage
+        kernel's time in ms.  The states are unchanged."""
         ms = ctypes.c_double()
         sh = (ctypes.c_int32 * 4)()
         _check(lib().dmx_batch_tune(self.h, ctypes.byref(ms), sh), "dmx_batch_tune")
