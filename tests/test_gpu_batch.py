@@ -180,12 +180,26 @@ def test_batch_tuned_shape(precision):
     b.close()
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("pair", [(P.LLF, P.LLF), (P.HLL, P.HLLA), (P.UPWIND, P.UPWIND), (P.HLLD, P.LLF)])
-def test_batch_solver_pairs(pair):
-    print(f"[gpu] batch solver pair {pair}", flush=True)
+def test_batch_solver_pairs(pair, fused):
+    """Every other Riemann solver pair, with the default shapes and with the
+    batched fused trace + face/edge kernel: the bits of single contexts in
+    float64."""
+    print(f"[gpu] batch solver pair {pair}, fused {fused}", flush=True)
     pbs = boxes(0, *pair)[:2]
-    ref = singles(pbs, 2)
+    shape = ((3, 8), (1, 16)) if fused else None
+    ref = []
+    for pb in pbs:
+        d = abi.Domain(pb.params, pb.uin)
+        if shape:
+            d.set_shape(*shape)
+        t, _ = d.advance(2, 0.0)
+        ref.append((t, d.download()))
+        d.close()
     doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    if shape:
+        doms[0].set_shape(*shape)
     b = abi.Batch(doms)
     b.advance(2, [0.0, 0.0])
     for d, (ts, us) in zip(doms, ref):
