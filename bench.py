@@ -70,11 +70,11 @@ def stage_bytes(group: str, wt: int, wb: int = 8) -> float:
 # that file; COUNTERS_SOURCE names the file and the commit whose kernels it
 # measured.
 COUNTERS_SOURCE = ("profiles/mhd_step_ot256_hbm_bytes.md, profiles/mhd_step_ot256_sq_counters.md, "
-                   "profiles/fused_trace_flux_ot256.md (kernels of commit 2735674)")
+                   "profiles/fused_trace_flux_ot256.md (kernels of commit 0f60dc7)")
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 570.7, "flux": 1943.7, "update_ct": 416.5, "compute_dt": 86.0,
+    "f64": {"trace": 570.6, "flux": 1876.2, "update_ct": 416.6, "compute_dt": 86.1,
             "boundary": 17.7},
-    "f32": {"fused": 234.5, "update_ct": 291.3, "compute_dt": 69.6, "boundary": 14.8},
+    "f32": {"fused": 234.5, "update_ct": 291.3, "compute_dt": 69.5, "boundary": 14.8},
 }
 
 
@@ -84,10 +84,10 @@ MEASURED_TRAFFIC = {
 # the bound that applies where the HBM fraction is low (bit-exact float64
 # division and square root)
 MEASURED_VALU_BUSY = {
-    "f64": {"trace": 0.504, "flux": 0.695, "update_ct": 0.352, "compute_dt": 0.729, "boundary": 0.047,
+    "f64": {"trace": 0.507, "flux": 0.702, "update_ct": 0.349, "compute_dt": 0.731, "boundary": 0.046,
             "fused": 0.502},
-    "f32": {"trace": 0.413, "flux": 0.640, "update_ct": 0.206, "compute_dt": 0.391, "boundary": 0.055,
-            "fused": 0.609},
+    "f32": {"trace": 0.413, "flux": 0.640, "update_ct": 0.206, "compute_dt": 0.389, "boundary": 0.056,
+            "fused": 0.607},
 }
 
 
@@ -121,7 +121,7 @@ def roofline(prof: dict, cells: int, prec: str) -> dict | None:
 # planes), (trace mode, planes)); --no-tune applies them without timing, so
 # that profiler runs measure the kernels of the timed run (under a profiler
 # the autotuner's timings, and so its choice, can change)
-TUNED_256 = {0: ((0, 4), (1, 16)), 1: ((3, 16), (1, 32))}
+TUNED_256 = {0: ((0, 2), (1, 16)), 1: ((3, 16), (1, 32))}
 # --fused: the trace and the face/edge stage in one kernel (mode 3, the trace
 # summary kept in LDS), 16 planes per block, in place of the tuned pair
 FUSED_SHAPE = ((3, 16), (1, 16))

@@ -186,8 +186,8 @@ const char* dmx_kernel_name(int32_t id);
 /* Launch-shape autotuner: on the uploaded state, times the candidate shapes
  * of the trace -- mode 0: a 16x4x4 cell tile per block; mode 1: a 32x8
  * column tile marching 8, 16 or 32 z-planes -- and of the face/edge Riemann
- * stage -- mode 0: one fused kernel marching 4, 8, 16 or 32 z-planes per
- * block; mode 1: separate face and edge kernels; mode 2: the fused kernel
+ * stage -- mode 0: one fused kernel marching 1, 2, 4, 8, 16 or 32 z-planes
+ * per block; mode 1: separate face and edge kernels; mode 2: the fused kernel
  * carrying the previous plane's states in LDS (8, 16 or 32 planes) -- keeps
  * the fastest of each, then times mode 3, one kernel for the trace and the
  * face/edge stage together (the trace summary stays in LDS; 8, 16 or 32
@@ -200,7 +200,8 @@ const char* dmx_kernel_name(int32_t id);
  * vector sizes by re-running the code. */
 int  dmx_tune(dmx_ctx* ctx, double* best_ms, int32_t* shape);
 /* Sets the launch shapes without timing them: shape[4] as dmx_tune returns
- * it (face/edge mode 0..3 with 4, 8, 16 or 32 planes, mode 2 not with 4;
+ * it (face/edge mode 0..3 with 4, 8, 16 or 32 planes, mode 2 not with 4,
+ * modes 0 and 1 also with 1 or 2;
  * trace mode 0, or 1 with 1..64 planes; mode 3 makes the trace shape
  * unused); for runs that must repeat a shape tuned before.  No counterpart
  * in the reference. */

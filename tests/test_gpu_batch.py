@@ -85,6 +85,35 @@ def test_batch_z_march_trace(precision, kt):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("kz", [1, 2])
+def test_batch_context_with_one_or_two_planes(precision, kz):
+    """Context 0 with the face/edge kernel over 1 or 2 planes per block
+    (the batch runs it over 4): single contexts with that shape and the
+    batch agree bit for bit in both precisions, since which block computes
+    a cell does not change its value."""
+    print(f"[gpu] batch, context 0 with the face/edge kernel over {kz} planes, precision {precision}", flush=True)
+    pbs = boxes(precision)
+    shape = ((0, kz), (1, 8))
+    ref = []
+    for pb in pbs:
+        d = abi.Domain(pb.params, pb.uin)
+        d.set_shape(*shape)
+        t, _ = d.advance(4, 0.0)
+        ref.append((t, d.download()))
+        d.close()
+    doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    doms[0].set_shape(*shape)
+    b = abi.Batch(doms)
+    t = b.advance(4, [0.0] * len(doms))
+    for d, tb, (ts, us) in zip(doms, t, ref):
+        assert tb == ts
+        assert np.array_equal(d.download(), us)
+    b.close()
+    for d in doms:
+        d.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("kt", [8, 16, 32])
 def test_batch_fused_kernel(precision, kt):
     """The batch's fused trace + face/edge kernel (kb_fused, the body of

@@ -309,7 +309,8 @@ def test_set_shape():
     pb = P.random_smooth(20, 12, 14, seed=41)
     print("[gpu] set_shape", flush=True)
     out = []
-    for flux, trace in (((0, 8), (0, 4)), ((2, 16), (1, 5)), ((1, 4), (1, 32)), ((3, 8), (0, 4))):
+    for flux, trace in (((0, 8), (0, 4)), ((2, 16), (1, 5)), ((1, 4), (1, 32)), ((3, 8), (0, 4)),
+                        ((0, 1), (1, 8)), ((0, 2), (1, 16)), ((1, 2), (0, 4))):
         d = abi.Domain(pb.params, pb.uin)
         d.set_shape(flux, trace)
         d.advance(3, 0.0)
@@ -317,7 +318,8 @@ def test_set_shape():
         d.close()
     assert all(np.array_equal(out[0], o) for o in out[1:])
     d = abi.Domain(pb.params, pb.uin)
-    for flux, trace in (((4, 8), (0, 4)), ((0, 5), (0, 4)), ((2, 4), (0, 4)), ((0, 8), (1, 0)), ((0, 8), (2, 4))):
+    for flux, trace in (((4, 8), (0, 4)), ((0, 5), (0, 4)), ((2, 4), (0, 4)), ((0, 8), (1, 0)), ((0, 8), (2, 4)),
+                        ((2, 2), (0, 4)), ((3, 1), (0, 4)), ((0, 3), (0, 4))):
         with pytest.raises(abi.DmxError):
             d.set_shape(flux, trace)
     d.close()
