@@ -70,11 +70,11 @@ def stage_bytes(group: str, wt: int, wb: int = 8) -> float:
 # that file; COUNTERS_SOURCE names the file and the commit whose kernels it
 # measured.
 COUNTERS_SOURCE = ("profiles/mhd_step_ot256_hbm_bytes.md, profiles/mhd_step_ot256_sq_counters.md, "
-                   "profiles/fused_trace_flux_ot256.md (kernels of commit 0f60dc7)")
+                   "profiles/fused_trace_flux_ot256.md (kernel code of commit 0f60dc7; the files name the launch shapes)")
 MEASURED_TRAFFIC = {
-    "f64": {"trace": 570.6, "flux": 1876.2, "update_ct": 416.6, "compute_dt": 86.1,
+    "f64": {"trace": 570.6, "flux": 1876.8, "update_ct": 414.4, "compute_dt": 86.3,
             "boundary": 17.7},
-    "f32": {"fused": 234.5, "update_ct": 291.3, "compute_dt": 69.5, "boundary": 14.8},
+    "f32": {"fused": 234.5, "update_ct": 295.0, "compute_dt": 69.5, "boundary": 14.8},
 }
 
 
@@ -84,10 +84,10 @@ MEASURED_TRAFFIC = {
 # the bound that applies where the HBM fraction is low (bit-exact float64
 # division and square root)
 MEASURED_VALU_BUSY = {
-    "f64": {"trace": 0.507, "flux": 0.702, "update_ct": 0.349, "compute_dt": 0.731, "boundary": 0.046,
+    "f64": {"trace": 0.504, "flux": 0.702, "update_ct": 0.371, "compute_dt": 0.730, "boundary": 0.049,
             "fused": 0.502},
-    "f32": {"trace": 0.413, "flux": 0.640, "update_ct": 0.206, "compute_dt": 0.389, "boundary": 0.056,
-            "fused": 0.607},
+    "f32": {"trace": 0.413, "flux": 0.640, "update_ct": 0.232, "compute_dt": 0.401, "boundary": 0.057,
+            "fused": 0.608},
 }
 
 
