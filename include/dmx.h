@@ -190,7 +190,7 @@ const char* dmx_kernel_name(int32_t id);
  * per block; mode 1: separate face and edge kernels; mode 2: the fused kernel
  * carrying the previous plane's states in LDS (8, 16 or 32 planes) -- keeps
  * the fastest of each, then times mode 3, one kernel for the trace and the
- * face/edge stage together (the trace summary stays in LDS; 8, 16 or 32
+ * face/edge stage together (the trace summary stays in LDS; 4, 8, 16 or 32
  * planes), against the best separate pair and keeps it if faster.  Returns
  * the face/edge stage's time per launch (mode 3: trace included) in
  * *best_ms and {face/edge mode, planes, trace mode, planes} in shape[4]
