@@ -85,6 +85,29 @@ def test_batch_z_march_trace(precision, kt):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+def test_update_planes_single_vs_batch(precision):
+    """A single context's update marches fewer planes per block where nz is
+    small (2 at nz 131, 4 at nz 261, 1 below 128; ragged last blocks), the
+    batch always 8: the same bits in both precisions, with the next dt
+    fused into the update (periodic) and without it (zero-gradient z)."""
+    pbs = [P.random_smooth(12, 10, 131, seed=51, precision=precision),
+           P.random_smooth(9, 8, 261, seed=52, precision=precision),
+           P.random_smooth(11, 9, 131, seed=53, precision=precision,
+                           bdtype=(P.PERIODIC, P.PERIODIC, P.ZERO_GRADIENT))]
+    print(f"[gpu] update planes, single contexts against a batch, precision {precision}", flush=True)
+    ref = singles(pbs, 3)
+    doms = [abi.Domain(pb.params, pb.uin) for pb in pbs]
+    b = abi.Batch(doms)
+    t = b.advance(3, [0.0] * len(doms))
+    for d, tb, (ts, us) in zip(doms, t, ref):
+        assert tb == ts
+        assert np.array_equal(d.download(), us)
+    b.close()
+    for d in doms:
+        d.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("kz", [1, 2])
 def test_batch_context_with_one_or_two_planes(precision, kz):
     """Context 0 with the face/edge kernel over 1 or 2 planes per block
